@@ -1,0 +1,69 @@
+/*
+ * gsknn_hip.h -- extensions of libgsknn.so beyond the reference C API.
+ *
+ * None of these has a counterpart in ChenhanYu/rnn; they give callers that
+ * keep their data on the GPU a stream-ordered form of the same operations,
+ * plus error reporting (the reference returns void and exits on failure,
+ * frame/gsknn_util.c:128-133).
+ *
+ * Conventions: every pointer passed to a *_dev function is a device pointer
+ * (hipMalloc); `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream); the call only enqueues work and returns 0, or a nonzero error
+ * code with gsknn_last_error() set.  Naming of m/n/amap/bmap/D/I is that of
+ * dgsknn in gsknn.h: m candidates (XA, amap), n queries (XB, bmap), heap row
+ * j of stride ldk belongs to query bmap[j] -- or to row hrow[j] when a heap
+ * row map is given.  d is the dimension and k the neighbour count.
+ */
+#ifndef GSKNN_MI355X_GSKNN_HIP_H
+#define GSKNN_MI355X_GSKNN_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Contraction selection for the FP64 path. */
+#define GSKNN_MODE_AUTO 0  /* MFMA when d >= 32, exact VALU below */
+#define GSKNN_MODE_EXACT 1 /* VALU, reference oracle arithmetic, bit-exact */
+#define GSKNN_MODE_MFMA 2  /* FP64 MFMA contraction */
+
+/* Error codes. */
+#define GSKNN_OK 0
+#define GSKNN_EINVAL 1   /* bad argument */
+#define GSKNN_EHIP 2     /* HIP runtime error */
+#define GSKNN_ENOMEM 3   /* device allocation failed */
+#define GSKNN_ELIMIT 4   /* size beyond what the kernels support */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *gsknn_last_error(void);
+
+/* Device used by the host-pointer API of gsknn.h (default 0). */
+int gsknn_set_device(int device);
+
+/* FP64 / FP32 kNN on device arrays (dgsknn semantics, canonical ties).
+ * `work` must hold gsknn_[ds]knn_work_bytes(...) bytes, or be NULL, in which
+ * case the call allocates stream-ordered scratch itself. */
+size_t gsknn_dknn_work_bytes(int m, int n, int d, int k, int mode);
+int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA,
+                   const double *XA2, const int *amap, const double *XB,
+                   const double *XB2, const int *bmap, double *D, int *I,
+                   int ldk, const int *hrow, int mode, void *work,
+                   size_t work_bytes, void *stream);
+
+size_t gsknn_sknn_work_bytes(int m, int n, int d, int k);
+int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
+                   const float *XA2, const int *amap, const float *XB,
+                   const float *XB2, const int *bmap, float *D, int *I,
+                   int ldk, const int *hrow, void *work, size_t work_bytes,
+                   void *stream);
+
+/* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
+int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);
+int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSKNN_MI355X_GSKNN_HIP_H */

@@ -1,0 +1,59 @@
+// Shared declarations of the HIP exact-kNN library (device and host side).
+//
+// Terminology follows the GSKNN C API (ChenhanYu/rnn include/gsknn.h:90-116):
+// m candidates gathered from table XA through amap, n queries gathered from
+// table XB through bmap; heap row j (stride ldk) belongs to query bmap[j] and
+// stores distances D and candidate indices I = amap values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gsknn {
+
+// Kernel arguments of one kNN launch.  All pointers are device pointers.
+template <typename T>
+struct KnnArgs {
+  int m, n, d, k, ldk;
+  const T *XA, *XA2;
+  const int *amap;
+  const T *XB, *XB2;
+  const int *bmap;
+  T *D;
+  int *I;
+  const int *hrow;  // heap row of query j (nullptr: row j)
+  int nsplit;       // candidate splits (grid.y); > 1 writes partial lists
+  int cps;          // candidates per split
+  T *partD;         // [nsplit][k][n] ascending partial lists (nsplit > 1)
+  int *partI;
+};
+
+enum Mode {
+  MODE_AUTO = 0,   // FP64: MFMA when d >= 32, else exact VALU
+  MODE_EXACT = 1,  // VALU, the reference oracle's arithmetic (bit-exact)
+  MODE_MFMA = 2,   // FP64 MFMA contraction
+};
+
+// Launchers (gsknn_kernels.hip).  Return hipSuccess or the launch error.
+hipError_t launch_knn_f64(const KnnArgs<double> &a, int mode, hipStream_t s,
+                          void *work, size_t work_bytes);
+hipError_t launch_knn_f32(const KnnArgs<float> &a, int mode, hipStream_t s,
+                          void *work, size_t work_bytes);
+// Workspace bytes the launch above needs (partial lists of split launches).
+size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
+
+// dgssq2nrm: C[i + j*ldc] for candidate i < m, query j < n.
+hipError_t launch_sq2nrm_f64(int m, int n, int d, const double *XA, const double *XA2,
+                             const int *amap, const double *XB, const double *XB2,
+                             const int *bmap, double *C, int ldc, hipStream_t s);
+// Header-layout rows (k > 512, 3-slot header + 4-ary heap) <-> plain rows.
+hipError_t launch_hdr_to_rows(int n, int k, int ldk, const double *Dh, const int *Ih,
+                              double *Dt, int *It, hipStream_t s);
+hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int *It,
+                              double *Dh, int *Ih, hipStream_t s);
+
+hipError_t launch_sqnorm_f64(int N, int d, const double *X, double *X2, hipStream_t s);
+hipError_t launch_sqnorm_f32(int N, int d, const float *X, float *X2, hipStream_t s);
+
+}  // namespace gsknn

@@ -1,0 +1,811 @@
+// HIP kernels of the exact-kNN path for gfx950 (CDNA4).
+//
+// Reference behaviour (ChenhanYu/rnn):
+//   distance  = (-2 * sum_p a_p b_p + XA2[amap[i]]) + XB2[bmap[j]], no clamp
+//               (frame/dgsknn_ref_impl.h:63-72, :91-103)
+//   selection = per heap row, keep the k smallest in a max-heap whose root is
+//               the current k-th distance (frame/gsknn_heap.c:104-162)
+//
+// Selection here uses the canonical total order (distance, index): a candidate
+// enters a row only when it is strictly smaller than the row's root, so a tie
+// in distance goes to the lower index and the result does not depend on the
+// order in which candidates are visited.  That is what lets a launch split the
+// candidates over workgroups and merge the partial lists afterwards.
+//
+// Kernels
+//   knn_exact_kernel<T>     one thread per query, VALU, the reference oracle's
+//                           arithmetic in p order (product and sum rounded
+//                           separately): distances bit-identical to it.
+//   knn_mfma_f64_kernel     FP64 MFMA (v_mfma_f64_16x16x4_f64) contraction,
+//                           wave-owned queries, LDS heaps.
+//   knn_merge_kernel<T>     folds the partial lists of a candidate-split
+//                           launch into the resident heap rows.
+//   sqnorm_kernel<T>        X2[i] = sum_p X[i][p]^2 in p order.
+//
+// Heaps live in LDS as [entry][query] so that lanes touching the same entry
+// of different queries hit different banks.  Rows are written back sorted
+// in descending order, which is a valid max-heap.
+#include <limits.h>
+
+#include "gsknn_common.h"
+
+namespace gsknn {
+
+__host__ __device__ constexpr size_t al16(size_t x) { return (x + 15) & ~size_t(15); }
+
+template <typename T>
+__device__ __forceinline__ bool cless(T da, int ia, T db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// Rounded product and sum, never fused (the library is built with
+// -ffp-contract=off as well).
+__device__ __forceinline__ double xmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double xadd(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float xmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float xadd(float a, float b) { return __fadd_rn(a, b); }
+
+// Sift (xd, xi) down from slot s of the heap of query q (k entries, LDS,
+// stride QS between entries).
+template <typename T>
+__device__ void sift_down(T *hd, int *hi, int QS, int q, int s, int k, T xd, int xi) {
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    T cd = hd[c * QS + q];
+    int ci = hi[c * QS + q];
+    if (c + 1 < k) {
+      T rd = hd[(c + 1) * QS + q];
+      int ri = hi[(c + 1) * QS + q];
+      if (cless(cd, ci, rd, ri)) {
+        ++c;
+        cd = rd;
+        ci = ri;
+      }
+    }
+    if (!cless(xd, xi, cd, ci)) break;
+    hd[s * QS + q] = cd;
+    hi[s * QS + q] = ci;
+    s = c;
+  }
+  hd[s * QS + q] = xd;
+  hi[s * QS + q] = xi;
+}
+
+template <typename T>
+__device__ __forceinline__ int heap_row(const KnnArgs<T> &a, int j) {
+  return a.hrow ? a.hrow[j] : j;
+}
+
+// Bring the resident row of query j into the LDS heap of slot q.  A launch
+// over all candidates heapifies the row under the canonical order; a split
+// launch starts from k copies of the row's canonical maximum instead, so only
+// candidates that beat the resident row end up in its partial list.
+template <typename T>
+__device__ void heap_load(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, int j,
+                          bool sentinel) {
+  const int k = a.k;
+  const size_t row = (size_t)heap_row(a, j) * a.ldk;
+  const T *Dr = a.D + row;
+  const int *Ir = a.I + row;
+  if (!sentinel) {
+    for (int t = 0; t < k; ++t) {
+      hd[t * QS + q] = Dr[t];
+      hi[t * QS + q] = Ir[t];
+    }
+    for (int s = k / 2 - 1; s >= 0; --s) sift_down(hd, hi, QS, q, s, k, hd[s * QS + q], hi[s * QS + q]);
+  } else {
+    T md = Dr[0];
+    int mi = Ir[0];
+    for (int t = 1; t < k; ++t) {
+      T x = Dr[t];
+      int xi = Ir[t];
+      if (cless(md, mi, x, xi)) {
+        md = x;
+        mi = xi;
+      }
+    }
+    for (int t = 0; t < k; ++t) {
+      hd[t * QS + q] = md;
+      hi[t * QS + q] = mi;
+    }
+  }
+}
+
+// Heapsort the LDS heap ascending in place, then write it out: descending
+// into the resident row, or ascending into the partial list of a split.
+template <typename T>
+__device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, int j, int split) {
+  const int k = a.k;
+  for (int e = k - 1; e > 0; --e) {
+    T xd = hd[e * QS + q];
+    int xi = hi[e * QS + q];
+    hd[e * QS + q] = hd[q];
+    hi[e * QS + q] = hi[q];
+    sift_down(hd, hi, QS, q, 0, e, xd, xi);
+  }
+  if (a.nsplit == 1) {
+    const size_t row = (size_t)heap_row(a, j) * a.ldk;
+    for (int t = 0; t < k; ++t) {
+      a.D[row + t] = hd[(k - 1 - t) * QS + q];
+      a.I[row + t] = hi[(k - 1 - t) * QS + q];
+    }
+  } else {
+    for (int t = 0; t < k; ++t) {
+      const size_t o = ((size_t)split * k + t) * a.n + j;
+      a.partD[o] = hd[t * QS + q];
+      a.partI[o] = hi[t * QS + q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact kernel: one thread per query, the reference oracle's arithmetic.
+// LDS: heap [k][QPB] | query rows [QPB][d+1] | candidate tile [d][CT] |
+//      candidate norms [CT] | candidate ids [CT]
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ExactLayout {
+  size_t hd, hi, sq, sc, cn, ci, bytes;
+  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT) {
+    hd = 0;
+    hi = al16(hd + sizeof(T) * (size_t)k * QPB);
+    sq = al16(hi + sizeof(int) * (size_t)k * QPB);
+    sc = al16(sq + sizeof(T) * (size_t)QPB * (d + 1));
+    cn = al16(sc + sizeof(T) * (size_t)d * CT);
+    ci = al16(cn + sizeof(T) * (size_t)CT);
+    bytes = al16(ci + sizeof(int) * (size_t)CT);
+  }
+};
+
+template <typename T, int QPB>
+__global__ __launch_bounds__(QPB) void knn_exact_kernel(KnnArgs<T> a, int CT) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k, d = a.d;
+  const ExactLayout<T> L(k, QPB, d, CT);
+  T *hd = (T *)(smem + L.hd);
+  int *hi = (int *)(smem + L.hi);
+  T *sq = (T *)(smem + L.sq);
+  T *sc = (T *)(smem + L.sc);
+  T *cn = (T *)(smem + L.cn);
+  int *ci = (int *)(smem + L.ci);
+
+  const int q = threadIdx.x;
+  const int j = blockIdx.x * QPB + q;
+  const bool valid = j < a.n;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+
+  T qn = T(0);
+  T rd = T(0);
+  int ri = 0;
+  T *myq = sq + (size_t)q * (d + 1);
+  if (valid) {
+    const T *xq = a.XB + (size_t)a.bmap[j] * d;
+    for (int p = 0; p < d; ++p) myq[p] = xq[p];
+    qn = a.XB2[a.bmap[j]];
+    heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
+    rd = hd[q];
+    ri = hi[q];
+  }
+
+  auto consider = [&](T s, int c) {
+    // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
+    const T dist = xadd(xadd(xmul(s, T(-2)), cn[c]), qn);
+    const int idx = ci[c];
+    if (cless(dist, idx, rd, ri)) {
+      sift_down(hd, hi, QPB, q, 0, k, dist, idx);
+      rd = hd[q];
+      ri = hi[q];
+    }
+  };
+
+  for (int c0 = cbeg; c0 < cend; c0 += CT) {
+    const int cnt = min(CT, cend - c0);
+    __syncthreads();
+    for (int e = q; e < cnt * d; e += QPB) {
+      const int c = e / d, p = e - c * d;
+      sc[p * CT + c] = a.XA[(size_t)a.amap[c0 + c] * d + p];
+    }
+    for (int e = q; e < cnt; e += QPB) {
+      const int id = a.amap[c0 + e];
+      cn[e] = a.XA2[id];
+      ci[e] = id;
+    }
+    __syncthreads();
+    if (!valid) continue;
+    int c = 0;
+    for (; c + 4 <= cnt; c += 4) {
+      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+      for (int p = 0; p < d; ++p) {
+        const T qp = myq[p];
+        const T *cp = sc + p * CT + c;
+        s0 = xadd(s0, xmul(cp[0], qp));
+        s1 = xadd(s1, xmul(cp[1], qp));
+        s2 = xadd(s2, xmul(cp[2], qp));
+        s3 = xadd(s3, xmul(cp[3], qp));
+      }
+      consider(s0, c);
+      consider(s1, c + 1);
+      consider(s2, c + 2);
+      consider(s3, c + 3);
+    }
+    for (; c < cnt; ++c) {
+      T s = T(0);
+      for (int p = 0; p < d; ++p) s = xadd(s, xmul(sc[p * CT + c], myq[p]));
+      consider(s, c);
+    }
+  }
+  if (valid) heap_store(a, hd, hi, QPB, q, j, split);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA kernel.
+//
+// A workgroup of W waves owns Q = W * 16 * QB queries; wave w owns 16 * QB of
+// them.  Candidates stream through LDS in tiles of C = 16 * CB, double
+// buffered, stored in MFMA fragment order so that each A-operand read is 64
+// consecutive doubles.  v_mfma_f64_16x16x4_f64 takes A = 16 candidates x 4
+// dims (lane l: candidate l%16, dim l/16) and B = 4 dims x 16 queries (lane l:
+// dim l/16, query l%16); lane l of the 16x16 result holds query l%16 against
+// candidates 4*(l/16) + v, v = 0..3.  The query fragments stay in registers
+// for the whole launch (d <= 4*KS).
+//
+// Epilogue per lane: dist = (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]], test
+// against the row's root.  A wave-wide ballot skips the insertion step when
+// no lane has a survivor, which is the common case once the rows are warm.
+// Otherwise the four lane groups (l/16 = 0..3, each covering the 16 queries
+// once) insert in turn, so no two lanes ever touch the same heap at once.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+struct MfmaLayout {
+  size_t tA, tn, ti, hd, hi, bytes;
+  __host__ __device__ MfmaLayout(int KS, int CB, int Q, int k) {
+    const size_t tile = (size_t)KS * CB * 64;
+    const size_t C = 16 * (size_t)CB;
+    tA = 0;
+    tn = al16(tA + sizeof(double) * 2 * tile);
+    ti = al16(tn + sizeof(double) * 2 * C);
+    hd = al16(ti + sizeof(int) * 2 * C);
+    hi = al16(hd + sizeof(double) * (size_t)k * Q);
+    bytes = al16(hi + sizeof(int) * (size_t)k * Q);
+  }
+};
+
+template <int KS, int QB, int CB, int W>
+__global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, DP = 4 * KS, NT = 64 * W;
+  constexpr int TILE = KS * CB * 64;
+  constexpr int LPT = (C * DP + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k, d = a.d;
+  const MfmaLayout L(KS, CB, Q, k);
+  double *tA = (double *)(smem + L.tA);
+  double *tn = (double *)(smem + L.tn);
+  int *ti = (int *)(smem + L.ti);
+  double *hd = (double *)(smem + L.hd);
+  int *hi = (int *)(smem + L.hi);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, grp = lane >> 4;
+  const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const bool sentinel = a.nsplit > 1;
+
+  if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
+
+  double bq[QB][KS];
+  double qn[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + (lane & 15);
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const double *xq = a.XB + (size_t)(qv[qb] ? a.bmap[j] : 0) * d;
+    qn[qb] = qv[qb] ? a.XB2[a.bmap[j]] : 0.0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int p = 4 * s + grp;
+      bq[qb][s] = (qv[qb] && p < d) ? xq[p] : 0.0;
+    }
+  }
+
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
+  double stage[LPT];
+  auto load_tile = [&](int t) {
+    const int c0 = cbeg + t * C;
+    const int cnt = min(C, cend - c0);
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) {
+      const int e = tid + u * NT;
+      double v = 0.0;
+      if (e < C * DP) {
+        const int c = e / DP, p = e - (e / DP) * DP;
+        if (c < cnt && p < d) v = a.XA[(size_t)a.amap[c0 + c] * d + p];
+      }
+      stage[u] = v;
+    }
+  };
+  auto store_tile = [&](int t, int buf) {
+    const int c0 = cbeg + t * C;
+    const int cnt = min(C, cend - c0);
+    double *A = tA + buf * TILE;
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) {
+      const int e = tid + u * NT;
+      if (e < C * DP) {
+        const int c = e / DP, p = e - (e / DP) * DP;
+        const int s = p >> 2, kk = p & 3, cb = c >> 4, i = c & 15;
+        A[(s * CB + cb) * 64 + i + 16 * kk] = stage[u];
+      }
+    }
+    if (tid < C) {
+      const bool ok = tid < cnt;
+      const int id = ok ? a.amap[c0 + tid] : INT_MAX;
+      ti[buf * C + tid] = id;
+      tn[buf * C + tid] = ok ? a.XA2[id] : 0.0;
+    }
+  };
+
+  __syncthreads();  // heaps loaded
+  double rd[QB];
+  int ri[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    rd[qb] = qv[qb] ? hd[ql[qb]] : 0.0;
+    ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
+  }
+
+  if (ntiles > 0) {
+    load_tile(0);
+    store_tile(0, 0);
+  }
+  __syncthreads();
+
+  for (int t = 0; t < ntiles; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < ntiles) load_tile(t + 1);
+
+    dbl4 acc[QB][CB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const double *A = tA + buf * TILE;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const double av = A[(s * CB + cb) * 64 + lane];
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+      }
+    }
+
+    const int c0 = cbeg + t * C;
+    const int cnt = min(C, cend - c0);
+    const double *TN = tn + buf * C;
+    const int *TI = ti + buf * C;
+    unsigned pass[QB];
+    bool anyp = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      pass[qb] = 0u;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = cb * 16 + 4 * grp + v;
+          const double dist = xadd(xadd(xmul(acc[qb][cb][v], -2.0), TN[c]), qn[qb]);
+          acc[qb][cb][v] = dist;
+          if (qv[qb] && c < cnt && cless(dist, TI[c], rd[qb], ri[qb])) {
+            pass[qb] |= 1u << (cb * 4 + v);
+            anyp = true;
+          }
+        }
+    }
+    if (__any(anyp)) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (grp == g) {
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            if (!pass[qb]) continue;
+            double r0 = hd[ql[qb]];
+            int r1 = hi[ql[qb]];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                if (pass[qb] & (1u << (cb * 4 + v))) {
+                  const int c = cb * 16 + 4 * grp + v;
+                  const double dist = acc[qb][cb][v];
+                  const int idx = TI[c];
+                  if (cless(dist, idx, r0, r1)) {
+                    sift_down(hd, hi, Q, ql[qb], 0, k, dist, idx);
+                    r0 = hd[ql[qb]];
+                    r1 = hi[ql[qb]];
+                  }
+                }
+              }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        rd[qb] = qv[qb] ? hd[ql[qb]] : 0.0;
+        ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
+      }
+    }
+    if (t + 1 < ntiles) store_tile(t + 1, buf ^ 1);
+    __syncthreads();
+  }
+
+  if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
+}
+
+// ---------------------------------------------------------------------------
+// Merge of split partial lists into the resident rows: one thread per query.
+// Each partial list is ascending, so the scan of a list stops at its first
+// entry that does not beat the root.  Sentinel entries (copies of the
+// resident maximum) never beat the root and are skipped by that rule.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void knn_merge_kernel(KnnArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k;
+  T *hd = (T *)smem;
+  int *hi = (int *)(smem + al16(sizeof(T) * (size_t)k * 64));
+  const int q = threadIdx.x;
+  const int j = blockIdx.x * 64 + q;
+  if (j >= a.n) return;
+  heap_load(a, hd, hi, 64, q, j, false);
+  T rd = hd[q];
+  int ri = hi[q];
+  for (int s = 0; s < a.nsplit; ++s) {
+    for (int t = 0; t < k; ++t) {
+      const size_t o = ((size_t)s * k + t) * a.n + j;
+      const T xd = a.partD[o];
+      const int xi = a.partI[o];
+      if (!cless(xd, xi, rd, ri)) break;
+      sift_down(hd, hi, 64, q, 0, k, xd, xi);
+      rd = hd[q];
+      ri = hi[q];
+    }
+  }
+  KnnArgs<T> b = a;
+  b.nsplit = 1;
+  heap_store(b, hd, hi, 64, q, j, 0);
+}
+
+template <typename T>
+__global__ void sqnorm_kernel(int N, int d, const T *X, T *X2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const T *x = X + (size_t)i * d;
+  T s = T(0);
+  for (int p = 0; p < d; ++p) s = xadd(s, xmul(x[p], x[p]));
+  X2[i] = s;
+}
+
+// dgssq2nrm: one thread per (candidate i, query j), the oracle's arithmetic.
+__global__ void sq2nrm_kernel(int m, int n, int d, const double *XA, const double *XA2,
+                              const int *amap, const double *XB, const double *XB2,
+                              const int *bmap, double *C, int ldc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  const double *a = XA + (size_t)amap[i] * d;
+  const double *b = XB + (size_t)bmap[j] * d;
+  double s = 0.0;
+  for (int p = 0; p < d; ++p) s = xadd(s, xmul(a[p], b[p]));
+  C[(size_t)j * ldc + i] = xadd(xadd(xmul(s, -2.0), XA2[amap[i]]), XB2[bmap[j]]);
+}
+
+// Header rows (frame/gsknn_heap.c:294-308): D[0] = ro, I[0] = k, I[1] = fill
+// count, I[2] = ldk, entries from offset 3.  While a row is filling, its
+// filled entries occupy the last `fill` slots (gsknn_heapselect_int_d4.c:
+// 66-75).  Unfilled slots become (+inf, INT_MAX) placeholders, which every
+// finite candidate beats.
+__global__ void hdr_to_rows_kernel(int n, int k, int ldk, const double *Dh, const int *Ih,
+                                   double *Dt, int *It) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (t >= k || j >= n) return;
+  const size_t h = (size_t)j * ldk;
+  const int fill = Ih[h + 1];
+  const bool real = fill >= k || t >= k - fill;
+  Dt[(size_t)j * k + t] = real ? Dh[h + 3 + t] : __builtin_inf();
+  It[(size_t)j * k + t] = real ? Ih[h + 3 + t] : INT_MAX;
+}
+
+// Rows come back sorted descending, placeholders first; the real entries go
+// to the last `count` slots, a valid 4-ary max-heap behind the header.
+__global__ void rows_to_hdr_kernel(int n, int k, int ldk, const double *Dt, const int *It,
+                                   double *Dh, int *Ih) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const size_t h = (size_t)j * ldk;
+  const double *dt = Dt + (size_t)j * k;
+  const int *it = It + (size_t)j * k;
+  int count = 0;
+  for (int t = 0; t < k; ++t) count += !(it[t] == INT_MAX && dt[t] == __builtin_inf());
+  for (int t = k - count; t < k; ++t) {
+    Dh[h + 3 + t] = dt[t];
+    Ih[h + 3 + t] = it[t];
+  }
+  Ih[h + 1] = count;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side planning and launch.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kLdsMax = 160 * 1024;  // gfx950 LDS per workgroup
+constexpr int kTargetBlocks = 512;      // 2 workgroups per CU on 256 CUs
+
+struct Plan {
+  int kind = 0;  // 0 exact, 1 mfma
+  int qpb = 64;  // queries per workgroup
+  int ct = 0;    // exact: candidates per tile
+  int ks = 0, qb = 0, w = 0;  // mfma
+  size_t lds = 0;
+  int gx = 0, nsplit = 1, cps = 0;
+};
+
+int exact_ct(int d, int elem) {
+  int ct = (int)((16384 / elem) / (d > 0 ? d : 1));
+  ct = ct < 4 ? 4 : (ct > 64 ? 64 : ct);
+  return ct & ~3;
+}
+
+int mfma_ks(int d) {
+  const int need = (d + 3) / 4;
+  for (int ks : {4, 8, 16, 32, 64})
+    if (need <= ks) return ks;
+  return 0;
+}
+
+constexpr int kCB = 2;
+
+Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
+  Plan p;
+  const bool f64 = elem == 8;
+  const int ks = f64 ? mfma_ks(d) : 0;
+  const bool mfma = f64 && ks > 0 && (mode == MODE_MFMA || (mode == MODE_AUTO && d >= 32));
+  int tile_c = 0;
+  if (mfma) {
+    p.kind = 1;
+    p.ks = ks;
+    const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
+    for (auto &o : opts) {
+      const int Q = o[1] * 16 * o[0];
+      const size_t b = MfmaLayout(ks, kCB, Q, k).bytes;
+      if (b <= kLdsMax) {
+        p.qb = o[0];
+        p.w = o[1];
+        p.qpb = Q;
+        p.lds = b;
+        break;
+      }
+    }
+    if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
+    tile_c = 16 * kCB;
+  }
+  if (p.kind == 0) {
+    p.ct = exact_ct(d, elem);
+    p.qpb = 0;
+    for (int qpb : {64, 32, 16}) {
+      const size_t b = f64 ? ExactLayout<double>(k, qpb, d, p.ct).bytes
+                           : ExactLayout<float>(k, qpb, d, p.ct).bytes;
+      if (b <= kLdsMax) {
+        p.qpb = qpb;
+        p.lds = b;
+        break;
+      }
+    }
+    tile_c = p.ct;
+  }
+  if (p.qpb == 0) return p;
+  p.gx = (n + p.qpb - 1) / p.qpb;
+  // Split the candidates when the query blocks alone cannot fill the chip;
+  // each split keeps at least 8 tiles of work.
+  int ns = 1;
+  if (p.gx < kTargetBlocks) {
+    ns = (kTargetBlocks + p.gx - 1) / p.gx;
+    const int maxs = m / (8 * tile_c);
+    if (ns > maxs) ns = maxs;
+    if (ns < 1) ns = 1;
+  }
+  if (ns > 1) {
+    int cps = (m + ns - 1) / ns;
+    cps = (cps + tile_c - 1) / tile_c * tile_c;
+    ns = (m + cps - 1) / cps;
+    p.cps = cps;
+    p.nsplit = ns;
+  }
+  if (p.nsplit <= 1) {
+    p.nsplit = 1;
+    p.cps = m;
+  }
+  return p;
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, size_t bytes) {
+  return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+template <int KS>
+hipError_t launch_mfma_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
+  const dim3 grid(p.gx, p.nsplit);
+  hipError_t e = hipSuccess;
+#define GSKNN_MFMA_CASE(QB_, W_)                                              \
+  if (p.qb == QB_ && p.w == W_) {                                             \
+    auto kern = knn_mfma_f64_kernel<KS, QB_, kCB, W_>;                        \
+    if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
+    return hipGetLastError();                                                 \
+  }
+  GSKNN_MFMA_CASE(2, 4)
+  GSKNN_MFMA_CASE(1, 4)
+  GSKNN_MFMA_CASE(1, 2)
+  GSKNN_MFMA_CASE(1, 1)
+#undef GSKNN_MFMA_CASE
+  return hipErrorInvalidConfiguration;
+}
+
+template <typename T>
+hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
+  const dim3 grid(p.gx, p.nsplit);
+  hipError_t e = hipSuccess;
+#define GSKNN_EXACT_CASE(QPB_)                                                \
+  if (p.qpb == QPB_) {                                                        \
+    auto kern = knn_exact_kernel<T, QPB_>;                                    \
+    if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
+    hipLaunchKernelGGL(kern, grid, dim3(QPB_), p.lds, s, a, p.ct);            \
+    return hipGetLastError();                                                 \
+  }
+  GSKNN_EXACT_CASE(64)
+  GSKNN_EXACT_CASE(32)
+  GSKNN_EXACT_CASE(16)
+#undef GSKNN_EXACT_CASE
+  return hipErrorInvalidConfiguration;
+}
+
+template <typename T>
+hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
+  const size_t lds = al16(sizeof(T) * (size_t)a.k * 64) + sizeof(int) * (size_t)a.k * 64;
+  if (lds > kLdsMax) return hipErrorInvalidConfiguration;
+  auto kern = knn_merge_kernel<T>;
+  hipError_t e = set_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((a.n + 63) / 64), dim3(64), lds, s, a);
+  return hipGetLastError();
+}
+
+size_t part_bytes(const Plan &p, int n, int k, int elem) {
+  if (p.nsplit <= 1) return 0;
+  const size_t ents = (size_t)p.nsplit * k * n;
+  return al16(ents * elem) + al16(ents * sizeof(int));
+}
+
+template <typename T>
+hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t work_bytes) {
+  if (a.m <= 0 || a.n <= 0 || a.d <= 0 || a.k <= 0) return hipSuccess;
+  const Plan p = make_plan(a.m, a.n, a.d, a.k, (int)sizeof(T), mode);
+  if (p.qpb == 0) return hipErrorInvalidConfiguration;
+  a.nsplit = p.nsplit;
+  a.cps = p.cps;
+  if (p.nsplit > 1) {
+    const size_t need = part_bytes(p, a.n, a.k, (int)sizeof(T));
+    if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
+    const size_t ents = (size_t)p.nsplit * a.k * a.n;
+    a.partD = (T *)work;
+    a.partI = (int *)((char *)work + al16(ents * sizeof(T)));
+  }
+  hipError_t e;
+  if (p.kind == 1) {
+    if constexpr (sizeof(T) == 8) {
+      switch (p.ks) {
+        case 4: e = launch_mfma_ks<4>(p, a, s); break;
+        case 8: e = launch_mfma_ks<8>(p, a, s); break;
+        case 16: e = launch_mfma_ks<16>(p, a, s); break;
+        case 32: e = launch_mfma_ks<32>(p, a, s); break;
+        case 64: e = launch_mfma_ks<64>(p, a, s); break;
+        default: return hipErrorInvalidConfiguration;
+      }
+    } else {
+      return hipErrorInvalidConfiguration;
+    }
+  } else {
+    e = launch_exact(p, a, s);
+  }
+  if (e != hipSuccess) return e;
+  if (p.nsplit > 1) e = launch_merge(a, s);
+  return e;
+}
+
+}  // namespace
+
+size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
+  if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return 0;
+  const Plan p = make_plan(m, n, d, k, elem_bytes, mode);
+  return part_bytes(p, n, k, elem_bytes);
+}
+
+hipError_t launch_knn_f64(const KnnArgs<double> &a, int mode, hipStream_t s, void *work,
+                          size_t work_bytes) {
+  return launch_knn<double>(a, mode, s, work, work_bytes);
+}
+
+hipError_t launch_knn_f32(const KnnArgs<float> &a, int mode, hipStream_t s, void *work,
+                          size_t work_bytes) {
+  return launch_knn<float>(a, mode, s, work, work_bytes);
+}
+
+hipError_t launch_sq2nrm_f64(int m, int n, int d, const double *XA, const double *XA2,
+                             const int *amap, const double *XB, const double *XB2,
+                             const int *bmap, double *C, int ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (n > 65535) {
+    for (int j0 = 0; j0 < n; j0 += 65535) {
+      const int nn = n - j0 < 65535 ? n - j0 : 65535;
+      hipError_t e = launch_sq2nrm_f64(m, nn, d, XA, XA2, amap, XB, XB2, bmap + j0,
+                                       C + (size_t)j0 * ldc, ldc, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(sq2nrm_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, d, XA, XA2,
+                     amap, XB, XB2, bmap, C, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_hdr_to_rows(int n, int k, int ldk, const double *Dh, const int *Ih,
+                              double *Dt, int *It, hipStream_t s) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  for (int j0 = 0; j0 < n; j0 += 65535) {
+    const int nn = n - j0 < 65535 ? n - j0 : 65535;
+    hipLaunchKernelGGL(hdr_to_rows_kernel, dim3((k + 255) / 256, nn), dim3(256), 0, s, nn, k, ldk,
+                       Dh + (size_t)j0 * ldk, Ih + (size_t)j0 * ldk, Dt + (size_t)j0 * k,
+                       It + (size_t)j0 * k);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int *It,
+                              double *Dh, int *Ih, hipStream_t s) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rows_to_hdr_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, k, ldk, Dt, It,
+                     Dh, Ih);
+  return hipGetLastError();
+}
+
+hipError_t launch_sqnorm_f64(int N, int d, const double *X, double *X2, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sqnorm_kernel<double>, dim3((N + 255) / 256), dim3(256), 0, s, N, d, X, X2);
+  return hipGetLastError();
+}
+
+hipError_t launch_sqnorm_f32(int N, int d, const float *X, float *X2, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sqnorm_kernel<float>, dim3((N + 255) / 256), dim3(256), 0, s, N, d, X, X2);
+  return hipGetLastError();
+}
+
+}  // namespace gsknn

@@ -1,0 +1,227 @@
+"""Python binding of libgsknn.so, mirroring the reference's python/gsknn.py.
+
+Same function names and argument order as ChenhanYu/rnn python/gsknn.py:
+``dgsknn(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I)`` and ``sgsknn(...)`` call
+``[ds]gsknn_var1`` with the query set first (python/gsknn.py:145-183), where
+``d`` is the dimension, ``k`` the neighbour count, ``q``/``r`` the query and
+reference (candidate) index arrays, and D/I the (n, k) result rows, updated in
+place.  The reference's size assert ``D.size == n * d`` (python/gsknn.py:157)
+confuses d and k; here the rows are checked against k.
+
+The reference also binds ``dgsknn_ref``/``sgsknn_ref``, its CPU oracle; those
+are not part of this library (the CPU checker lives in oracle/).
+
+The library is found next to this file (rnn_amd/lib/libgsknn.so) or under
+``$GSKNN_DIR/lib`` as in the reference (python/gsknn.py:32).  There is no
+fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgsknn.so")
+
+MODE_AUTO, MODE_EXACT, MODE_MFMA = 0, 1, 2
+
+_lib = None
+
+
+class GsknnError(RuntimeError):
+    pass
+
+
+def lib_path() -> str:
+    if os.path.exists(LIB_PATH):
+        return LIB_PATH
+    env = os.environ.get("GSKNN_DIR")
+    if env and os.path.exists(os.path.join(env, "lib", "libgsknn.so")):
+        return os.path.join(env, "lib", "libgsknn.so")
+    raise GsknnError(
+        f"libgsknn.so not found at {LIB_PATH}: build it with `python -m rnn_amd.build`")
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = ctypes.CDLL(lib_path())
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    L.gsknn_last_error.restype = ctypes.c_char_p
+    L.gsknn_dknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
+    L.gsknn_dknn_dev.restype = i32
+    L.gsknn_sknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, vp, sz, vp]
+    L.gsknn_sknn_dev.restype = i32
+    L.gsknn_dknn_work_bytes.argtypes = [i32] * 5
+    L.gsknn_dknn_work_bytes.restype = sz
+    L.gsknn_sknn_work_bytes.argtypes = [i32] * 4
+    L.gsknn_sknn_work_bytes.restype = sz
+    L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
+    L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
+    for name in ("dgsknn_var1", "sgsknn_var1"):
+        getattr(L, name).argtypes = [i32] * 4 + [vp] * 8
+        getattr(L, name).restype = None
+    for name in ("dgsknn", "sgsknn", "dgsknn_var3"):
+        getattr(L, name).argtypes = [i32] * 4 + [vp] * 7
+        getattr(L, name).restype = None
+    L.dgssq2nrm.argtypes = [i32] * 3 + [vp] * 7 + [i32]
+    L.heapCreate_d.argtypes = [i32, i32, ctypes.c_double]
+    L.heapCreate_d.restype = ctypes.POINTER(HeapT)
+    L.heapCreate_s.argtypes = [i32, i32, ctypes.c_float]
+    L.heapCreate_s.restype = ctypes.POINTER(HeapT)
+    L.heapAttach_d.argtypes = [i32, i32, vp, vp]
+    L.heapAttach_d.restype = ctypes.POINTER(HeapT)
+    L.heapAttach_s.argtypes = [i32, i32, vp, vp]
+    L.heapAttach_s.restype = ctypes.POINTER(HeapT)
+    L.heapFree_d.argtypes = [vp]
+    L.heapFree_s.argtypes = [vp]
+    L.heapSelect_d.argtypes = [i32, i32, vp, vp, vp, vp]
+    L.heapSelect_s.argtypes = [i32, i32, vp, vp, vp, vp]
+    L.gsknn_malloc_aligned.argtypes = [i32, i32, i32]
+    L.gsknn_malloc_aligned.restype = vp
+
+
+class HeapT(ctypes.Structure):
+    """heap_t, byte-identical to the reference (include/gsknn.h:75-88)."""
+    _fields_ = [("m", ctypes.c_int), ("k", ctypes.c_int), ("ldk", ctypes.c_int),
+                ("d", ctypes.c_int), ("ro", ctypes.c_double),
+                ("D", ctypes.POINTER(ctypes.c_double)), ("ro_s", ctypes.c_float),
+                ("D_s", ctypes.POINTER(ctypes.c_float)), ("I", ctypes.POINTER(ctypes.c_int)),
+                ("type", ctypes.c_int), ("prec", ctypes.c_int)]
+
+
+def last_error() -> str:
+    return lib().gsknn_last_error().decode()
+
+
+def _raise_if_error():
+    msg = last_error()
+    if msg:
+        raise GsknnError(msg)
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _check(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I, dtype):
+    # python/gsknn.py:60-80, with the row check taken against k.
+    M, N = Xr.shape[0], Xq.shape[0]
+    m, n = r.size, q.size
+    assert Xr.shape[1] == Xq.shape[1] == d
+    assert M >= m and N >= n
+    assert Xq2.size == N and Xr2.size == M
+    assert D.size == n * k and I.size == n * k
+    for a in (Xq, Xr, Xq2, Xr2, D):
+        assert a.dtype == dtype and a.flags.c_contiguous
+    for a in (q, r, I):
+        assert a.dtype == np.int32 and a.flags.c_contiguous
+    return m, n
+
+
+def dgsknn(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I):
+    """python/gsknn.py:145-183: FP64 kNN of queries Xq[q] among Xr[r]."""
+    m, n = _check(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I, np.float64)
+    lib().dgsknn_var1(n, m, d, k, _p(Xq), _p(Xq2), _p(q), _p(Xr), _p(Xr2), _p(r), _p(D), _p(I))
+    _raise_if_error()
+
+
+def sgsknn(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I):
+    """python/gsknn.py:55-95: FP32 kNN of queries Xq[q] among Xr[r]."""
+    m, n = _check(d, k, Xq, Xq2, q, Xr, Xr2, r, D, I, np.float32)
+    lib().sgsknn_var1(n, m, d, k, _p(Xq), _p(Xq2), _p(q), _p(Xr), _p(Xr2), _p(r), _p(D), _p(I))
+    _raise_if_error()
+
+
+# ---------------------------------------------------------------------------
+# C-API style calls on numpy arrays (dgsknn/sgsknn argument order).
+# ---------------------------------------------------------------------------
+def dgsknn_heap(XA, XA2, amap, XB, XB2, bmap, D, I):
+    """dgsknn(m, n, d, k, XA, XA2, amap, XB, XB2, bmap, heap) on (n, k) rows
+    attached as a heap_t (frame/gsknn.c:1138-1184)."""
+    n, k = D.shape
+    h = lib().heapAttach_d(n, k, _p(D), _p(I))
+    try:
+        lib().dgsknn(amap.size, bmap.size, XA.shape[1], k, _p(XA), _p(XA2), _p(amap), _p(XB),
+                     _p(XB2), _p(bmap), ctypes.cast(h, ctypes.c_void_p))
+        _raise_if_error()
+    finally:
+        _free_shell(h)
+
+
+def sgsknn_heap(XA, XA2, amap, XB, XB2, bmap, D, I):
+    n, k = D.shape
+    h = lib().heapAttach_s(n, k, _p(D), _p(I))
+    try:
+        lib().sgsknn(amap.size, bmap.size, XA.shape[1], k, _p(XA), _p(XA2), _p(amap), _p(XB),
+                     _p(XB2), _p(bmap), ctypes.cast(h, ctypes.c_void_p))
+        _raise_if_error()
+    finally:
+        _free_shell(h)
+
+
+def _free_shell(h):
+    # heapFree_[sd] frees D/I too (frame/gsknn_heap.c:321-337); the arrays here
+    # belong to numpy, so only the heap_t shell is released.
+    ctypes.CDLL(None).free(ctypes.cast(h, ctypes.c_void_p))
+
+
+# ---------------------------------------------------------------------------
+# Device-array calls (torch tensors on the GPU; no host staging).
+# ---------------------------------------------------------------------------
+def _dp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def dknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work=None,
+             stream=None):
+    """FP64 kNN on device tensors, enqueued on ``stream`` (torch's current
+    stream by default).  D/I: (rows, ldk) device tensors."""
+    k = D.shape[1] if hrow is None else D.shape[1]
+    m, n, d = amap.numel(), bmap.numel(), XA.shape[1]
+    wb = work.numel() * work.element_size() if work is not None else 0
+    rc = lib().gsknn_dknn_dev(m, n, d, k, _dp(XA), _dp(XA2), _dp(amap), _dp(XB), _dp(XB2),
+                              _dp(bmap), _dp(D), _dp(I), D.stride(0), _dp(hrow), mode,
+                              _dp(work), wb, _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+def sknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, work=None, stream=None):
+    k = D.shape[1]
+    m, n, d = amap.numel(), bmap.numel(), XA.shape[1]
+    wb = work.numel() * work.element_size() if work is not None else 0
+    rc = lib().gsknn_sknn_dev(m, n, d, k, _dp(XA), _dp(XA2), _dp(amap), _dp(XB), _dp(XB2),
+                              _dp(bmap), _dp(D), _dp(I), D.stride(0), _dp(hrow), _dp(work), wb,
+                              _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+def work_bytes_d(m, n, d, k, mode=MODE_AUTO) -> int:
+    return int(lib().gsknn_dknn_work_bytes(m, n, d, k, mode))
+
+
+def work_bytes_s(m, n, d, k) -> int:
+    return int(lib().gsknn_sknn_work_bytes(m, n, d, k))
+
+
+def sqnorm_dev(X, X2, stream=None):
+    fn = lib().gsknn_dsqnorm_dev if X.element_size() == 8 else lib().gsknn_ssqnorm_dev
+    rc = fn(X.shape[0], X.shape[1], _dp(X), _dp(X2), _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())

@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+
+
+@pytest.fixture(scope="session")
+def fixtures():
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", "gsknn_ref_fixtures.npz")
+    with np.load(path, allow_pickle=False) as z:
+        data = {k: z[k] for k in z.files}
+    cases = {}
+    for key, val in data.items():
+        name, field = key.split("__")
+        cases.setdefault(name, {})[field] = val
+    return cases

@@ -1,0 +1,287 @@
+"""GPU parity of libgsknn.so against the oracle, through the C-ABI.
+
+Bars (neighbour indices exact; distances per precision):
+  * exact kernels (FP64 d < 32, every FP32 call): distances bit-identical to
+    the oracle, which is itself bit-identical to the reference's dgsknn_ref;
+  * FP64 MFMA kernel: |D - D_ref| <= 1e-12 * max(|D_ref|, |q|^2 + |r|^2).
+Inputs without exact ties are compared with the reference's own tie rule
+(the oracle in compat mode); inputs with exact ties (integer lattices) with
+the canonical (distance, lower index) rule.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rnn_amd import datasets as ds
+from rnn_amd import gsknn
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def _sorted(D, I):
+    return oracle.sort_rows(np.ascontiguousarray(D), np.ascontiguousarray(I))
+
+
+def assert_rows(D, I, Dref, Iref, X2q=None, X2c_rows=None, tol=None):
+    """Indices exact after sorting by (dist, idx); distances bitwise or within tol."""
+    Ds, Is = _sorted(D, I)
+    Dr, Ir = _sorted(Dref, Iref)
+    np.testing.assert_array_equal(Is, Ir)
+    if tol is None:
+        assert np.array_equal(Ds.view(np.uint8), Dr.view(np.uint8)), \
+            f"max |dD| = {np.nanmax(np.abs(Ds - Dr))}"
+    else:
+        scale = np.maximum(np.abs(Dr), X2q[:, None] + X2c_rows)
+        bad = np.abs(Ds - Dr) > tol * scale
+        assert not bad.any(), f"{bad.sum()} distances beyond tolerance"
+
+
+def run_gpu(X, X2, amaps, bmap, D, I):
+    for am in amaps:
+        if X.dtype == np.float64:
+            gsknn.dgsknn_heap(X, X2, am, X, X2, bmap, D, I)
+        else:
+            gsknn.sgsknn_heap(X, X2, am, X, X2, bmap, D, I)
+
+
+def run_oracle(X, X2, amaps, bmap, D, I, canonical=False):
+    for am in amaps:
+        oracle.gsknn(X, X2, am, X, X2, bmap, D, I, canonical=canonical)
+
+
+@pytest.mark.parametrize("name", ["c1small", "partial", "repeats", "odd", "d1", "radius", "f32"])
+def test_reference_fixtures(fixtures, name):
+    c = fixtures[name]
+    print(f"[gpu] fixture {name}", flush=True)
+    amaps = [c["amap"]] + [c[f"amap{i}"] for i in range(2, 5) if f"amap{i}" in c]
+    D, I = c["D0"].copy(), c["I0"].copy()
+    for i, am in enumerate(amaps, start=1):
+        run_gpu(c["X"], c["X2"], [am], c["bmap"], D, I)
+        assert_rows(D, I, c[f"D{i}"], c[f"I{i}"])
+        assert oracle.rows_are_heaps(D, I)
+
+
+def test_lattice_ties_lower_index_wins(fixtures):
+    c = fixtures["lattice"]
+    X, X2, am, bm = c["X"], c["X2"], c["amap"], c["bmap"]
+    for mode in ("exact", "mfma"):
+        print(f"[gpu] lattice {mode}", flush=True)
+        D, I = c["D0"].copy(), c["I0"].copy()
+        Dr, Ir = c["D0"].copy(), c["I0"].copy()
+        import torch
+        dev = torch.device("cuda")
+        t = {k: torch.from_numpy(v).to(dev) for k, v in
+             dict(X=X, X2=X2, am=am, bm=bm, D=D, I=I).items()}
+        gsknn.dknn_dev(t["X"], t["X2"], t["am"], t["X"], t["X2"], t["bm"], t["D"], t["I"],
+                       mode=gsknn.MODE_EXACT if mode == "exact" else gsknn.MODE_MFMA)
+        torch.cuda.synchronize()
+        run_oracle(X, X2, [am], bm, Dr, Ir, canonical=True)
+        assert_rows(t["D"].cpu().numpy(), t["I"].cpu().numpy(), Dr, Ir)
+
+
+def test_config1_full_both_calls():
+    p = ds.config1()
+    print("[gpu] config1 N=8192 d=16 m=n=2048 k=16", flush=True)
+    D, I = ds.empty_heap(p.n, p.k)
+    Dr, Ir = D.copy(), I.copy()
+    for am in [p.amap] + p.extra_amaps:
+        gsknn.dgsknn_heap(p.XA, p.XA2, am, p.XB, p.XB2, p.bmap, D, I)
+        oracle.gsknn(p.XA, p.XA2, am, p.XB, p.XB2, p.bmap, Dr, Ir)
+        assert_rows(D, I, Dr, Ir)
+
+
+def test_config1_literal_second_call():
+    # The other reading of the second call: a disjoint query set into the
+    # same heap rows.
+    p = ds.config1()
+    pi = np.random.Generator(np.random.PCG64(0)).permutation(8192).astype(np.int32)
+    D, I = ds.empty_heap(p.n, p.k)
+    Dr, Ir = D.copy(), I.copy()
+    for bm in (p.bmap, pi[2048:4096].copy()):
+        gsknn.dgsknn_heap(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, D, I)
+        oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, Dr, Ir)
+        assert_rows(D, I, Dr, Ir)
+
+
+@pytest.mark.parametrize("mode", ["auto", "exact"])
+def test_config2_scaled(mode, monkeypatch):
+    # Config-2 generator at 1/8 scale (N=16384, m=n=8192, d=64, k=32) with the
+    # 1024-candidate prefill call.
+    p = ds.config2(scale=8)
+    print(f"[gpu] config2/8 mode={mode}", flush=True)
+    import torch
+    dev = torch.device("cuda")
+    X = torch.from_numpy(p.XA).to(dev)
+    X2 = torch.from_numpy(p.XA2).to(dev)
+    D, I = ds.empty_heap(p.n, p.k)
+    Dr, Ir = D.copy(), I.copy()
+    tD, tI = torch.from_numpy(D).to(dev), torch.from_numpy(I).to(dev)
+    bm = torch.from_numpy(p.bmap).to(dev)
+    m = gsknn.MODE_AUTO if mode == "auto" else gsknn.MODE_EXACT
+    for am in (p.prefill_amap, p.amap):
+        gsknn.dknn_dev(X, X2, torch.from_numpy(am).to(dev), X, X2, bm, tD, tI, mode=m)
+        torch.cuda.synchronize()
+        oracle.gsknn(p.XA, p.XA2, am, p.XB, p.XB2, p.bmap, Dr, Ir)
+        Dg, Ig = tD.cpu().numpy(), tI.cpu().numpy()
+        if mode == "exact":
+            assert_rows(Dg, Ig, Dr, Ir)
+        else:
+            Is = _sorted(Dr, Ir)[1]
+            x2c = np.where(Is >= 0, p.XA2[np.maximum(Is, 0)], 0.0)
+            assert_rows(Dg, Ig, Dr, Ir, X2q=p.XB2[p.bmap], X2c_rows=x2c, tol=TOL64)
+
+
+@pytest.mark.parametrize("d", [16, 64, 100])
+def test_candidate_split_path(d):
+    # Few queries, many candidates: the launch splits the candidates over
+    # workgroups and merges partial lists.
+    g = np.random.Generator(np.random.PCG64(d))
+    X = g.standard_normal((30000, d))
+    X2 = oracle.sqnorm(X)
+    amap = g.integers(0, 30000, 25000).astype(np.int32)
+    bmap = g.integers(0, 30000, 70).astype(np.int32)
+    D, I = ds.empty_heap(70, 20)
+    Dr, Ir = D.copy(), I.copy()
+    print(f"[gpu] split d={d}", flush=True)
+    gsknn.dgsknn_heap(X, X2, amap, X, X2, bmap, D, I)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir)
+    if d < 32:
+        assert_rows(D, I, Dr, Ir)
+    else:
+        Is = _sorted(Dr, Ir)[1]
+        assert_rows(D, I, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+
+
+def test_sgsknn_fp32_bitwise():
+    g = np.random.Generator(np.random.PCG64(42))
+    X = g.standard_normal((6000, 24)).astype(np.float32)
+    X2 = oracle.sqnorm(X)
+    amap = g.permutation(6000)[:4000].astype(np.int32)
+    bmap = g.permutation(6000)[:900].astype(np.int32)
+    D, I = ds.empty_heap(900, 17, np.float32)
+    Dr, Ir = D.copy(), I.copy()
+    print("[gpu] sgsknn fp32", flush=True)
+    gsknn.sgsknn_heap(X, X2, amap, X, X2, bmap, D, I)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir)
+    assert_rows(D, I, Dr, Ir)
+
+
+def test_python_binding_mirror():
+    # python/gsknn.py argument order: (d, k, Xq, Xq2, q, Xr, Xr2, r, D, I).
+    g = np.random.Generator(np.random.PCG64(1))
+    Xq = g.random((300, 6))
+    Xr = g.random((500, 6))
+    Xq2, Xr2 = oracle.sqnorm(Xq), oracle.sqnorm(Xr)
+    q = np.arange(0, 300, 3, dtype=np.int32)
+    r = np.arange(0, 500, 2, dtype=np.int32)
+    D = np.full((q.size, 5), 1.79e308)
+    I = np.full((q.size, 5), -1, dtype=np.int32)
+    gsknn.dgsknn(6, 5, Xq, Xq2, q, Xr, Xr2, r, D, I)
+    Dr, Ir = np.full_like(D, 1.79e308), np.full_like(I, -1)
+    oracle.gsknn(Xr, Xr2, r, Xq, Xq2, q, Dr, Ir)
+    assert_rows(D, I, Dr, Ir)
+    Ds = D.astype(np.float32)
+    Xq32, Xr32 = Xq.astype(np.float32), Xr.astype(np.float32)
+    Xq232, Xr232 = oracle.sqnorm(Xq32), oracle.sqnorm(Xr32)
+    Ds = np.full((q.size, 5), 1.79e30, dtype=np.float32)
+    Is = np.full((q.size, 5), -1, dtype=np.int32)
+    gsknn.sgsknn(6, 5, Xq32, Xq232, q, Xr32, Xr232, r, Ds, Is)
+    Dr, Ir = np.full_like(Ds, 1.79e30), np.full_like(Is, -1)
+    oracle.gsknn(Xr32, Xr232, r, Xq32, Xq232, q, Dr, Ir)
+    assert_rows(Ds, Is, Dr, Ir)
+
+
+def test_device_api_with_heap_row_map():
+    import torch
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(8))
+    X = g.random((2000, 40))
+    X2 = oracle.sqnorm(X)
+    amap = g.permutation(2000)[:1500].astype(np.int32)
+    bmap = g.permutation(2000)[:300].astype(np.int32)
+    hrow = g.permutation(500)[:300].astype(np.int32)  # rows scattered in a 500-row pool
+    k = 12
+    pool_D = np.full((500, k), np.inf)
+    pool_I = np.full((500, k), -1, dtype=np.int32)
+    tD, tI = torch.from_numpy(pool_D).to(dev), torch.from_numpy(pool_I).to(dev)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    gsknn.dknn_dev(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2,
+                   torch.from_numpy(bmap).to(dev), tD, tI,
+                   hrow=torch.from_numpy(hrow).to(dev), mode=gsknn.MODE_EXACT)
+    torch.cuda.synchronize()
+    Dg, Ig = tD.cpu().numpy(), tI.cpu().numpy()
+    Dr, Ir = ds.empty_heap(300, k)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir)
+    assert_rows(Dg[hrow], Ig[hrow], Dr, Ir)
+    untouched = np.setdiff1d(np.arange(500), hrow)
+    assert np.all(Ig[untouched] == -1)
+
+
+def test_large_k_header_layout_var3():
+    # r > 512 goes through dgsknn_var3 on heapCreate_d rows with the 3-slot
+    # header and a 4-ary heap (frame/gsknn_heap.c:257-308).
+    g = np.random.Generator(np.random.PCG64(13))
+    X = g.random((3000, 8))
+    X2 = oracle.sqnorm(X)
+    L = gsknn.lib()
+    k, n = 600, 40
+    bmap = g.permutation(3000)[:n].astype(np.int32)
+    h = L.heapCreate_d(n, k, 1.79e308)
+    ldk = h.contents.ldk
+    Dr, Ir = ds.empty_heap(n, k)
+    Dr[:] = 1.79e308
+    for m in (250, 1200):  # first call leaves rows partially filled
+        amap = g.permutation(3000)[:m].astype(np.int32)
+        print(f"[gpu] var3 k={k} m={m}", flush=True)
+        L.dgsknn(m, n, 8, k, gsknn._p(X), gsknn._p(X2), gsknn._p(amap), gsknn._p(X), gsknn._p(X2),
+                 gsknn._p(bmap), ctypes.cast(h, ctypes.c_void_p))
+        assert gsknn.last_error() == ""
+        oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
+        Dh = np.ctypeslib.as_array(h.contents.D, shape=(n * ldk,)).reshape(n, ldk)
+        Ih = np.ctypeslib.as_array(h.contents.I, shape=(n * ldk,)).reshape(n, ldk)
+        fill = Ih[:, 1]
+        assert np.all(Ih[:, 0] == k) and np.all(Ih[:, 2] == ldk)
+        Ds, Is = _sorted(Dh[:, 3:3 + k], Ih[:, 3:3 + k])
+        Drs, Irs = _sorted(Dr, Ir)
+        np.testing.assert_array_equal(Is, Irs)
+        assert np.array_equal(Ds, Drs)
+        assert np.all(fill == min(k, 250 if m == 250 else k))
+    L.heapFree_d(ctypes.cast(h, ctypes.c_void_p))
+
+
+def test_sq2nrm_matrix():
+    g = np.random.Generator(np.random.PCG64(21))
+    X = g.random((400, 9))
+    X2 = oracle.sqnorm(X)
+    amap = g.permutation(400)[:70].astype(np.int32)
+    bmap = g.permutation(400)[:50].astype(np.int32)
+    ldc = 72
+    C = np.full(ldc * 50, -7.0)
+    L = gsknn.lib()
+    L.dgssq2nrm(70, 50, 9, gsknn._p(X), gsknn._p(X2), gsknn._p(amap), gsknn._p(X), gsknn._p(X2),
+                gsknn._p(bmap), gsknn._p(C), ldc)
+    C = C.reshape(50, ldc)
+    A, B = X[amap], X[bmap]
+    s = np.zeros((50, 70))
+    for p in range(9):
+        s = s + B[:, p:p + 1] * A[:, p][None, :]
+    ref = (s * -2.0 + X2[amap][None, :]) + X2[bmap][:, None]
+    assert np.array_equal(C[:, :70], ref)
+    assert np.all(C[:, 70:] == -7.0)
+
+
+def test_deterministic_and_empty():
+    p = ds.config1(scale=4)
+    D1, I1 = ds.empty_heap(p.n, p.k)
+    D2, I2 = D1.copy(), I1.copy()
+    gsknn.dgsknn_heap(p.XA, p.XA2, p.amap, p.XB, p.XB2, p.bmap, D1, I1)
+    gsknn.dgsknn_heap(p.XA, p.XA2, p.amap, p.XB, p.XB2, p.bmap, D2, I2)
+    assert np.array_equal(D1, D2) and np.array_equal(I1, I2)
+    D3, I3 = D1.copy(), I1.copy()
+    gsknn.dgsknn_heap(p.XA, p.XA2, p.amap[:0].copy(), p.XB, p.XB2, p.bmap, D3, I3)
+    assert np.array_equal(D1, D3) and np.array_equal(I1, I3)
