@@ -95,7 +95,7 @@ static int knn_dev(const char *fn, int m, int n, int d, int k, const T *XA, cons
   else
     e = launch_knn_f32(a, mode, s, work, work_bytes);
   if (own) (void)hipFreeAsync(own, s);
-  if (e == hipErrorInvalidConfiguration)
+  if (e == hipErrorNotSupported)
     return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d k=%d", fn, d, k);
   if (e != hipSuccess) return hip_err(e, fn);
   return GSKNN_OK;
@@ -332,7 +332,7 @@ int knn_host(const char *fn, int m, int n, int d, int k, const T *XA, const T *X
   hipError_t e = launch_any(a, mode, s, work, wb);
   if (e != hipSuccess) {
     (void)hipStreamSynchronize(s);
-    if (e == hipErrorInvalidConfiguration)
+    if (e == hipErrorNotSupported)
       return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d k=%d", fn, d, k);
     return hip_err(e, fn);
   }
@@ -380,7 +380,7 @@ int var3_host(int m, int n, int d, int r, const double *XA, const double *XA2,
   if (e == hipSuccess) e = launch_rows_to_hdr(n, r, ldk, (const double *)tD, (const int *)tI, gD, gI, s);
   if (e != hipSuccess) {
     (void)hipStreamSynchronize(s);
-    if (e == hipErrorInvalidConfiguration)
+    if (e == hipErrorNotSupported)
       return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d r=%d", fn, d, r);
     return hip_err(e, fn);
   }

@@ -249,8 +249,8 @@ __global__ __launch_bounds__(QPB) void knn_exact_kernel(KnnArgs<T> a, int CT) {
 // consecutive doubles.  v_mfma_f64_16x16x4_f64 takes A = 16 candidates x 4
 // dims (lane l: candidate l%16, dim l/16) and B = 4 dims x 16 queries (lane l:
 // dim l/16, query l%16); lane l of the 16x16 result holds query l%16 against
-// candidates 4*(l/16) + v, v = 0..3.  The query fragments stay in registers
-// for the whole launch (d <= 4*KS).
+// candidates l/16 + 4*v, v = 0..3 (measured on gfx950).  The query fragments
+// stay in registers for the whole launch (d <= 4*KS).
 //
 // Epilogue per lane: dist = (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]], test
 // against the row's root.  A wave-wide ballot skips the insertion step when
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
       for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int c = cb * 16 + 4 * grp + v;
+          const int c = cb * 16 + grp + 4 * v;
           const double dist = xadd(xadd(xmul(acc[qb][cb][v], -2.0), TN[c]), qn[qb]);
           acc[qb][cb][v] = dist;
           if (qv[qb] && c < cnt && cless(dist, TI[c], rd[qb], ri[qb])) {
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
                 if (pass[qb] & (1u << (cb * 4 + v))) {
-                  const int c = cb * 16 + 4 * grp + v;
+                  const int c = cb * 16 + grp + 4 * v;
                   const double dist = acc[qb][cb][v];
                   const int idx = TI[c];
                   if (cless(dist, idx, r0, r1)) {
@@ -460,16 +460,16 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
 // entry that does not beat the root.  Sentinel entries (copies of the
 // resident maximum) never beat the root and are skipped by that rule.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(64) void knn_merge_kernel(KnnArgs<T> a) {
+template <typename T, int QPB>
+__global__ __launch_bounds__(QPB) void knn_merge_kernel(KnnArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k;
   T *hd = (T *)smem;
-  int *hi = (int *)(smem + al16(sizeof(T) * (size_t)k * 64));
+  int *hi = (int *)(smem + al16(sizeof(T) * (size_t)k * QPB));
   const int q = threadIdx.x;
-  const int j = blockIdx.x * 64 + q;
+  const int j = blockIdx.x * QPB + q;
   if (j >= a.n) return;
-  heap_load(a, hd, hi, 64, q, j, false);
+  heap_load(a, hd, hi, QPB, q, j, false);
   T rd = hd[q];
   int ri = hi[q];
   for (int s = 0; s < a.nsplit; ++s) {
@@ -478,14 +478,14 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(KnnArgs<T> a) {
       const T xd = a.partD[o];
       const int xi = a.partI[o];
       if (!cless(xd, xi, rd, ri)) break;
-      sift_down(hd, hi, 64, q, 0, k, xd, xi);
+      sift_down(hd, hi, QPB, q, 0, k, xd, xi);
       rd = hd[q];
       ri = hi[q];
     }
   }
   KnnArgs<T> b = a;
   b.nsplit = 1;
-  heap_store(b, hd, hi, 64, q, j, 0);
+  heap_store(b, hd, hi, QPB, q, j, 0);
 }
 
 template <typename T>
@@ -579,6 +579,16 @@ int mfma_ks(int d) {
 
 constexpr int kCB = 2;
 
+size_t merge_lds(int k, int elem, int qpb) {
+  return al16((size_t)elem * k * qpb) + sizeof(int) * (size_t)k * qpb;
+}
+
+int merge_qpb(int k, int elem) {
+  for (int qpb : {64, 32, 16})
+    if (merge_lds(k, elem, qpb) <= kLdsMax) return qpb;
+  return 0;
+}
+
 Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   Plan p;
   const bool f64 = elem == 8;
@@ -622,7 +632,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   // Split the candidates when the query blocks alone cannot fill the chip;
   // each split keeps at least 8 tiles of work.
   int ns = 1;
-  if (p.gx < kTargetBlocks) {
+  if (p.gx < kTargetBlocks && merge_qpb(k, elem) > 0) {
     ns = (kTargetBlocks + p.gx - 1) / p.gx;
     const int maxs = m / (8 * tile_c);
     if (ns > maxs) ns = maxs;
@@ -664,7 +674,7 @@ hipError_t launch_mfma_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s
   GSKNN_MFMA_CASE(1, 2)
   GSKNN_MFMA_CASE(1, 1)
 #undef GSKNN_MFMA_CASE
-  return hipErrorInvalidConfiguration;
+  return hipErrorNotSupported;
 }
 
 template <typename T>
@@ -682,18 +692,26 @@ hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   GSKNN_EXACT_CASE(32)
   GSKNN_EXACT_CASE(16)
 #undef GSKNN_EXACT_CASE
-  return hipErrorInvalidConfiguration;
+  return hipErrorNotSupported;
 }
 
 template <typename T>
 hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
-  const size_t lds = al16(sizeof(T) * (size_t)a.k * 64) + sizeof(int) * (size_t)a.k * 64;
-  if (lds > kLdsMax) return hipErrorInvalidConfiguration;
-  auto kern = knn_merge_kernel<T>;
-  hipError_t e = set_lds(kern, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((a.n + 63) / 64), dim3(64), lds, s, a);
-  return hipGetLastError();
+  const int qpb = merge_qpb(a.k, (int)sizeof(T));
+  const size_t lds = merge_lds(a.k, (int)sizeof(T), qpb);
+  hipError_t e = hipSuccess;
+#define GSKNN_MERGE_CASE(QPB_)                                                \
+  if (qpb == QPB_) {                                                          \
+    auto kern = knn_merge_kernel<T, QPB_>;                                    \
+    if ((e = set_lds(kern, lds)) != hipSuccess) return e;                     \
+    hipLaunchKernelGGL(kern, dim3((a.n + QPB_ - 1) / QPB_), dim3(QPB_), lds, s, a); \
+    return hipGetLastError();                                                 \
+  }
+  GSKNN_MERGE_CASE(64)
+  GSKNN_MERGE_CASE(32)
+  GSKNN_MERGE_CASE(16)
+#undef GSKNN_MERGE_CASE
+  return hipErrorNotSupported;
 }
 
 size_t part_bytes(const Plan &p, int n, int k, int elem) {
@@ -706,7 +724,7 @@ template <typename T>
 hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t work_bytes) {
   if (a.m <= 0 || a.n <= 0 || a.d <= 0 || a.k <= 0) return hipSuccess;
   const Plan p = make_plan(a.m, a.n, a.d, a.k, (int)sizeof(T), mode);
-  if (p.qpb == 0) return hipErrorInvalidConfiguration;
+  if (p.qpb == 0) return hipErrorNotSupported;
   a.nsplit = p.nsplit;
   a.cps = p.cps;
   if (p.nsplit > 1) {
@@ -725,10 +743,10 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
         case 16: e = launch_mfma_ks<16>(p, a, s); break;
         case 32: e = launch_mfma_ks<32>(p, a, s); break;
         case 64: e = launch_mfma_ks<64>(p, a, s); break;
-        default: return hipErrorInvalidConfiguration;
+        default: return hipErrorNotSupported;
       }
     } else {
-      return hipErrorInvalidConfiguration;
+      return hipErrorNotSupported;
     }
   } else {
     e = launch_exact(p, a, s);

@@ -45,8 +45,20 @@ def lib_path() -> str:
 
 
 def lib() -> ctypes.CDLL:
+    """Load libgsknn.so.
+
+    PyTorch ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    libamdhip64.so.7).  Importing torch first makes the dynamic linker bind
+    this library's libamdhip64.so.7 dependency to that same copy, so device
+    pointers and streams from torch and from this library belong to one
+    runtime; loading the system runtime first would leave two in the process.
+    """
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(lib_path())
         _declare(_lib)
     return _lib
