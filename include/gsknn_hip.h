@@ -58,6 +58,29 @@ int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
                    int ldk, const int *hrow, void *work, size_t work_bytes,
                    void *stream);
 
+/* Exact kNN from a BF16 MFMA candidate pass and an FP64 re-rank.
+ *
+ * FP32 point tables XA (NA rows) and XB (NB rows) are first prepared once
+ * each with gsknn_bf16_prep_dev into a buffer of gsknn_bf16_prep_bytes(N, d)
+ * bytes (BF16 copy, FP64 squared norms, rounding-error norms).  The kNN call
+ * then bounds every BF16 distance by a rigorous error margin, keeps the
+ * candidates whose lower bound can still reach a row's k-th distance, and
+ * re-ranks them with FP64 direct differences sum_p (a_p - b_p)^2, merging
+ * into FP64 heap rows (D, I; stride ldk; optional row map hrow) under the
+ * canonical (distance, index) order.  The k-sets equal those of FP64
+ * distances.  `overflow` (device int, may be NULL) receives the number of
+ * queries whose candidate list overflowed and were re-scanned exactly.
+ * d <= 256. */
+size_t gsknn_bf16_prep_bytes(int N, int d);
+int gsknn_bf16_prep_dev(int N, int d, const float *X, void *prep, void *stream);
+size_t gsknn_bf16_work_bytes(int m, int n, int d, int k);
+int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
+                       const void *prepA, const int *amap, int NB,
+                       const float *XB, const void *prepB, const int *bmap,
+                       double *D, int *I, int ldk, const int *hrow,
+                       void *work, size_t work_bytes, int *overflow,
+                       void *stream);
+
 /* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
 int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);
 int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *stream);

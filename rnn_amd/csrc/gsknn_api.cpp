@@ -138,6 +138,54 @@ extern "C" int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *
   return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_ssqnorm_dev");
 }
 
+extern "C" size_t gsknn_bf16_prep_bytes(int N, int d) {
+  return (N > 0 && d > 0) ? bf16_prep_bytes(N, d) : 0;
+}
+
+extern "C" int gsknn_bf16_prep_dev(int N, int d, const float *X, void *prep, void *stream) {
+  if (N < 0 || d <= 0 || d > 256)
+    return set_err(GSKNN_EINVAL, "gsknn_bf16_prep_dev: N=%d d=%d (need 0 < d <= 256)", N, d);
+  if (N > 0 && (!X || !prep)) return set_err(GSKNN_EINVAL, "gsknn_bf16_prep_dev: null array");
+  hipError_t e = launch_bf16_prep(N, d, X, prep, (hipStream_t)stream);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_bf16_prep_dev");
+}
+
+extern "C" size_t gsknn_bf16_work_bytes(int m, int n, int d, int k) {
+  return (m > 0 && n > 0 && d > 0 && k > 0) ? bf16_work_bytes(m, n, d, k) : 0;
+}
+
+extern "C" int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
+                                  const void *prepA, const int *amap, int NB, const float *XB,
+                                  const void *prepB, const int *bmap, double *D, int *I, int ldk,
+                                  const int *hrow, void *work, size_t work_bytes, int *overflow,
+                                  void *stream) {
+  const char *fn = "gsknn_bf16_knn_dev";
+  int rc = check_sizes(fn, m, n, d, k, ldk);
+  if (rc) return rc;
+  if (m == 0 || n == 0 || d == 0 || k == 0) return GSKNN_OK;
+  if (d > 256) return set_err(GSKNN_ELIMIT, "%s: d=%d > 256", fn, d);
+  if (!XA || !prepA || !amap || !XB || !prepB || !bmap || !D || !I)
+    return set_err(GSKNN_EINVAL, "%s: null array", fn);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t need = bf16_work_bytes(m, n, d, k);
+  if (need == 0) return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d k=%d", fn, d, k);
+  void *own = nullptr;
+  if (work == nullptr || work_bytes < need) {
+    if (work != nullptr)
+      return set_err(GSKNN_EINVAL, "%s: workspace %zu < %zu bytes", fn, work_bytes, need);
+    hipError_t e = hipMallocAsync(&own, need, s);
+    if (e != hipSuccess) return hip_err(e, "hipMallocAsync(workspace)");
+    work = own;
+    work_bytes = need;
+  }
+  hipError_t e = launch_knn_bf16(m, n, d, k, NA, NB, XA, prepA, amap, XB, prepB, bmap, D, I, ldk,
+                                 hrow, work, work_bytes, overflow, s);
+  if (own) (void)hipFreeAsync(own, s);
+  if (e == hipErrorNotSupported)
+    return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d k=%d", fn, d, k);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, fn);
+}
+
 // ---------------------------------------------------------------------------
 // Host-pointer staging
 // ---------------------------------------------------------------------------

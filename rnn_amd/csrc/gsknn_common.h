@@ -53,6 +53,15 @@ hipError_t launch_hdr_to_rows(int n, int k, int ldk, const double *Dh, const int
 hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int *It,
                               double *Dh, int *Ih, hipStream_t s);
 
+// BF16 candidate pass + FP64 re-rank (gsknn_rerank.hip).
+size_t bf16_prep_bytes(int N, int d);
+hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_t s);
+size_t bf16_work_bytes(int m, int n, int d, int k);
+hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const float *XA,
+                           const void *prepA, const int *amap, const float *XB, const void *prepB,
+                           const int *bmap, double *D, int *I, int ldk, const int *hrow,
+                           void *work, size_t work_bytes, int *overflow_out, hipStream_t s);
+
 hipError_t launch_sqnorm_f64(int N, int d, const double *X, double *X2, hipStream_t s);
 hipError_t launch_sqnorm_f32(int N, int d, const float *X, float *X2, hipStream_t s);
 

@@ -1,0 +1,637 @@
+// BF16 candidate pass + FP64 re-rank: exact kNN from a BF16 MFMA contraction.
+//
+// No counterpart in the reference (whose kernels are FP64/FP32 only,
+// kernels/x86_64/sandybridge/); the result is the exact k-set of the FP64
+// distances, merged into dgsknn-style heap rows (frame/gsknn_heap.c:104-162
+// semantics, canonical (distance, index) ties).
+//
+// Inputs are FP32 point tables.  Preparation (once per table) stores, per row:
+//   xb   the row rounded to BF16 (round to nearest even), zero padded to DP,
+//   n64  sum_p x_p^2 in FP64,
+//   xn   |bf16(x)| and en = |x - bf16(x)|, both rounded up to FP32.
+//
+// Error bound.  For candidate a and query b with BF16 images a~, b~ and
+// residuals e_a = a - a~, e_b = b - b~ (exact in FP32):
+//   |a.b - a~.b~| <= |e_a||b~| + |a~||e_b| + |e_a||e_b|            (inputs)
+//   BF16 products are exact in FP32; an FP32 sum of d terms in any order,
+//   each addition rounding to within 2u (u = 2^-24), is off by at most
+//   g |a~||b~| with g = (d+1) 2u / (1 - (d+1) 2u)                    (sum)
+//   t = fma(-2, dot, nc) and t + nq round twice; the FP32 norms round once
+//   each: at most 4u (nq + nc + 2 |a~||b~|)                           (epilogue)
+// so with dist~ = (nq + nc) - 2 dot~ evaluated in FP32 and the true
+// distance D = |a - b|^2 = nq + nc - 2 a.b,
+//   |dist~ - D| <= delta = 2(en_a xn_b + xn_a en_b + en_a en_b)
+//                          + (2g + 8u) xn_a xn_b + 4u (nq + nc)
+// (times 1 + 2^-10 for the rounding of delta itself).  lower = dist~ - delta
+// and upper = dist~ + delta bracket D.
+//
+// Pass 1 (knn_bf16_filter_kernel): per query, U = k-th smallest of
+// {upper bounds of all candidates} U {resident heap distances}; U bounds the
+// final k-th distance from above.  Every candidate whose lower bound is <= U
+// at the time it is seen is appended to the query's list (position, lower).
+// Pass 2 (knn_rerank_kernel): one wave per query drops list entries with
+// lower > final U, computes sum_p (a_p - b_p)^2 in FP64 for the rest, and
+// merges them into the resident row under the canonical order.  A query
+// whose list overflowed is re-scanned against every candidate in FP64.
+#include <limits.h>
+
+#include "gsknn_common.h"
+
+namespace gsknn {
+
+namespace {
+
+__host__ __device__ constexpr size_t al16r(size_t x) { return (x + 15) & ~size_t(15); }
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kU32 = 5.9604644775390625e-08f;  // 2^-24
+
+__device__ __forceinline__ bool cless_d(double da, int ia, double db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+}  // namespace
+
+// Prepared-table layout: bf16 rows [N][DP] | n64 [N] | xn [N] | en [N] |
+// maxima {xn, en, n32} as 3 floats.
+struct PrepLayout {
+  size_t xb, n64, xn, en, mx, bytes;
+  int DP;
+  __host__ __device__ PrepLayout(int N, int d) {
+    DP = (d + 31) / 32 * 32;
+    xb = 0;
+    n64 = al16r(xb + (size_t)N * DP * 2);
+    xn = al16r(n64 + (size_t)N * 8);
+    en = al16r(xn + (size_t)N * 4);
+    mx = al16r(en + (size_t)N * 4);
+    bytes = al16r(mx + 16);
+  }
+};
+
+size_t bf16_prep_bytes(int N, int d) { return PrepLayout(N, d).bytes; }
+
+__global__ void prep_zero_kernel(float *mx) {
+  if (threadIdx.x < 4) mx[threadIdx.x] = 0.0f;
+}
+
+__global__ void prep_bf16_kernel(int N, int d, const float *X, unsigned char *prep) {
+  const PrepLayout L(N, d);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  __bf16 *xb = (__bf16 *)(prep + L.xb) + (size_t)i * L.DP;
+  const float *x = X + (size_t)i * d;
+  double s = 0.0, sb = 0.0, se = 0.0;
+  for (int p = 0; p < d; ++p) {
+    const float v = x[p];
+    const __bf16 hb = (__bf16)v;  // round to nearest even
+    const float vb = (float)hb;
+    const float e = v - vb;       // exact
+    xb[p] = hb;
+    s += (double)v * (double)v;
+    sb += (double)vb * (double)vb;
+    se += (double)e * (double)e;
+  }
+  for (int p = d; p < L.DP; ++p) xb[p] = (__bf16)0.0f;
+  ((double *)(prep + L.n64))[i] = s;
+  // sqrt in FP64 then round up to FP32; the 1 + 2^-40 covers the FP64 sum.
+  const float xn = __double2float_ru(sqrt(sb) * (1.0 + 0x1p-40));
+  const float en = __double2float_ru(sqrt(se) * (1.0 + 0x1p-40));
+  const float n32 = __double2float_ru(s);
+  ((float *)(prep + L.xn))[i] = xn;
+  ((float *)(prep + L.en))[i] = en;
+  float *mx = (float *)(prep + L.mx);
+  // Non-negative floats order like their bit patterns.
+  atomicMax((int *)&mx[0], __float_as_int(xn));
+  atomicMax((int *)&mx[1], __float_as_int(en));
+  atomicMax((int *)&mx[2], __float_as_int(n32));
+}
+
+hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  const PrepLayout L(N, d);
+  hipLaunchKernelGGL(prep_zero_kernel, dim3(1), dim3(64), 0, s, (float *)((char *)prep + L.mx));
+  hipLaunchKernelGGL(prep_bf16_kernel, dim3((N + 255) / 256), dim3(256), 0, s, N, d, X,
+                     (unsigned char *)prep);
+  return hipGetLastError();
+}
+
+struct Bf16Args {
+  int m, n, d, k, ldk, cap;
+  int NA, NB;                  // rows of the candidate / query tables
+  const float *XA, *XB;        // FP32 tables (re-rank)
+  const unsigned char *prepA, *prepB;
+  const int *amap, *bmap;
+  double *D;
+  int *I;
+  const int *hrow;
+  int2 *lst;    // [n][cap] (candidate position, lower bound bits)
+  int *cnt;     // [n]
+  float *U;     // [n]
+  int *nover;   // queries whose list overflowed
+};
+
+// Lane l of v_mfma_f32_16x16x32_bf16 holds 8 consecutive k of row l%16 of A
+// (and of column l%16 of B) starting at k = 8*(l/16); result entry v of lane l
+// is row kRowOf(l, v), column l%16.
+__device__ __forceinline__ int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+
+struct FilterLayout {
+  size_t tA, nc, xn, en, pos, heap, cnt, bytes;
+  __host__ __device__ FilterLayout(int KS, int CB, int Q, int k) {
+    const size_t tile = (size_t)KS * CB * 64 * 8;  // bf16 elements
+    const size_t C = 16 * (size_t)CB;
+    tA = 0;
+    nc = al16r(tA + 2 * tile * 2);
+    xn = al16r(nc + 2 * C * 4);
+    en = al16r(xn + 2 * C * 4);
+    pos = al16r(en + 2 * C * 4);
+    heap = al16r(pos + 2 * C * 4);
+    cnt = al16r(heap + (size_t)k * Q * 4);
+    bytes = al16r(cnt + (size_t)Q * 4);
+  }
+};
+
+__device__ void sift_vals(float *h, int QS, int q, int k, float x) {
+  int s = 0;
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    float cv = h[c * QS + q];
+    if (c + 1 < k) {
+      const float rv = h[(c + 1) * QS + q];
+      if (cv < rv) {
+        ++c;
+        cv = rv;
+      }
+    }
+    if (!(x < cv)) break;
+    h[s * QS + q] = cv;
+    s = c;
+  }
+  h[s * QS + q] = x;
+}
+
+template <int KS, int QB, int CB, int W>
+__global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, NT = 64 * W;
+  constexpr int DP = 32 * KS;
+  constexpr int TILE = KS * CB * 64 * 8;      // bf16 elements per tile
+  constexpr int CHUNKS = C * (DP / 8);        // 16-byte chunks per tile
+  constexpr int LPT = (CHUNKS + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k;
+  const FilterLayout L(KS, CB, Q, k);
+  __bf16 *tA = (__bf16 *)(smem + L.tA);
+  float *tnc = (float *)(smem + L.nc);
+  float *txn = (float *)(smem + L.xn);
+  float *ten = (float *)(smem + L.en);
+  int *tpos = (int *)(smem + L.pos);
+  float *hv = (float *)(smem + L.heap);
+  int *hc = (int *)(smem + L.cnt);
+
+  const PrepLayout PA(a.NA, a.d), PB(a.NB, a.d);
+  const __bf16 *xbA = (const __bf16 *)(a.prepA + PA.xb);
+  const __bf16 *xbB = (const __bf16 *)(a.prepB + PB.xb);
+  const double *n64A = (const double *)(a.prepA + PA.n64);
+  const double *n64B = (const double *)(a.prepB + PB.n64);
+  const float *xnA = (const float *)(a.prepA + PA.xn);
+  const float *xnB = (const float *)(a.prepB + PB.xn);
+  const float *enA = (const float *)(a.prepA + PA.en);
+  const float *enB = (const float *)(a.prepB + PB.en);
+  const float *mxA = (const float *)(a.prepA + PA.mx);
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4;
+  const int qbase = blockIdx.x * Q;
+
+  // Upper-bound heaps start from the resident rows (rounded up to FP32).
+  if (tid < Q) {
+    const int j = qbase + tid;
+    if (j < a.n) {
+      const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+      for (int t = 0; t < k; ++t) hv[t * Q + tid] = __double2float_ru(a.D[row + t]);
+      for (int s = k / 2 - 1; s >= 0; --s) {
+        // bottom-up heapify of a values-only max-heap
+        float x = hv[s * Q + tid];
+        int p = s;
+        while (true) {
+          int c = 2 * p + 1;
+          if (c >= k) break;
+          float cv = hv[c * Q + tid];
+          if (c + 1 < k && cv < hv[(c + 1) * Q + tid]) cv = hv[(++c) * Q + tid];
+          if (!(x < cv)) break;
+          hv[p * Q + tid] = cv;
+          p = c;
+        }
+        hv[p * Q + tid] = x;
+      }
+    }
+    hc[tid] = 0;
+  }
+
+  // Query fragments and per-query bound factors.
+  const float u = kU32;
+  const float g = (float)((a.d + 1) * 2.0 * 0x1p-24 / (1.0 - (a.d + 1) * 2.0 * 0x1p-24));
+  const float cxx = 2.0f * g + 8.0f * u;
+  bf16x8 bq[QB][KS];
+  float nq[QB], al[QB], be[QB], cq[QB], dmx[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + (lane & 15);
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const int row = qv[qb] ? a.bmap[j] : 0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+      bq[qb][s] = qv[qb] ? *(const bf16x8 *)(xbB + (size_t)row * DP + s * 32 + 8 * grp)
+                         : bf16x8{};
+    const float xq = qv[qb] ? xnB[row] : 0.0f;
+    const float eq = qv[qb] ? enB[row] : 0.0f;
+    nq[qb] = qv[qb] ? (float)n64B[row] : 0.0f;
+    al[qb] = 2.0f * eq + cxx * xq;
+    be[qb] = 2.0f * xq + 2.0f * eq;
+    cq[qb] = 4.0f * u * nq[qb];
+    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f;
+  }
+
+  const int ntiles = (a.m + C - 1) / C;
+  uint4 stage[LPT];
+  auto load_tile = [&](int t) {
+    const int c0 = t * C;
+    const int cnt = min(C, a.m - c0);
+#pragma unroll
+    for (int uu = 0; uu < LPT; ++uu) {
+      const int e = tid + uu * NT;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (e < CHUNKS) {
+        const int c = e / (DP / 8), ch = e % (DP / 8);
+        if (c < cnt) v = *(const uint4 *)(xbA + (size_t)a.amap[c0 + c] * DP + ch * 8);
+      }
+      stage[uu] = v;
+    }
+  };
+  auto store_tile = [&](int t, int buf) {
+    const int c0 = t * C;
+    const int cnt = min(C, a.m - c0);
+    __bf16 *A = tA + buf * TILE;
+#pragma unroll
+    for (int uu = 0; uu < LPT; ++uu) {
+      const int e = tid + uu * NT;
+      if (e < CHUNKS) {
+        const int c = e / (DP / 8), ch = e % (DP / 8);
+        const int s = ch >> 2, gg = ch & 3, cb = c >> 4, i = c & 15;
+        *(uint4 *)(A + ((size_t)(s * CB + cb) * 64 + i + 16 * gg) * 8) = stage[uu];
+      }
+    }
+    if (tid < C) {
+      const bool ok = tid < cnt;
+      const int id = ok ? a.amap[c0 + tid] : 0;
+      tnc[buf * C + tid] = ok ? (float)n64A[id] : __builtin_inff();
+      txn[buf * C + tid] = ok ? xnA[id] : 0.0f;
+      ten[buf * C + tid] = ok ? enA[id] : 0.0f;
+      tpos[buf * C + tid] = ok ? c0 + tid : -1;
+    }
+  };
+
+  __syncthreads();
+  float U[QB], thr[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    U[qb] = qv[qb] ? hv[ql[qb]] : -__builtin_inff();
+    thr[qb] = U[qb] - nq[qb] + dmx[qb];
+  }
+
+  if (ntiles > 0) {
+    load_tile(0);
+    store_tile(0, 0);
+  }
+  __syncthreads();
+
+  for (int t = 0; t < ntiles; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < ntiles) load_tile(t + 1);
+    f32x4 acc[QB][CB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const __bf16 *A = tA + buf * TILE;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const bf16x8 av = *(const bf16x8 *)(A + ((size_t)(s * CB + cb) * 64 + lane) * 8);
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+      }
+    }
+    const float *NC = tnc + buf * C;
+    unsigned pass[QB];
+    bool anyp = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      pass[qb] = 0u;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = cb * 16 + out_row(lane, v);
+          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], NC[c]);
+          acc[qb][cb][v] = tt;
+          if (tt <= thr[qb]) {
+            pass[qb] |= 1u << (cb * 4 + v);
+            anyp = true;
+          }
+        }
+    }
+    if (__any(anyp)) {
+      const float *XN = txn + buf * C;
+      const float *EN = ten + buf * C;
+      const int *PS = tpos + buf * C;
+#pragma unroll
+      for (int gsel = 0; gsel < 4; ++gsel) {
+        if (grp == gsel) {
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            if (!pass[qb]) continue;
+            const int slot = ql[qb];
+            const int j = qbase + slot;
+            float Uq = hv[slot];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                if (!(pass[qb] & (1u << (cb * 4 + v)))) continue;
+                const int c = cb * 16 + out_row(lane, v);
+                const int ps = PS[c];
+                if (ps < 0) continue;
+                const float dist = acc[qb][cb][v] + nq[qb];
+                const float delta =
+                    (XN[c] * al[qb] + EN[c] * be[qb] + 4.0f * u * NC[c] + cq[qb]) * 1.0009765625f +
+                    1e-30f;
+                const float lower = dist - delta;
+                const float upper = dist + delta;
+                if (lower <= Uq) {
+                  const int at = hc[slot]++;
+                  if (at < a.cap)
+                    a.lst[(size_t)j * a.cap + at] = make_int2(ps, __float_as_int(lower));
+                }
+                if (upper < Uq) {
+                  sift_vals(hv, Q, slot, k, upper);
+                  Uq = hv[slot];
+                }
+              }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        if (!qv[qb]) continue;
+        U[qb] = hv[ql[qb]];
+        thr[qb] = U[qb] - nq[qb] + dmx[qb];
+      }
+    }
+    if (t + 1 < ntiles) store_tile(t + 1, buf ^ 1);
+    __syncthreads();
+  }
+
+  if (tid < Q) {
+    const int j = qbase + tid;
+    if (j < a.n) {
+      a.U[j] = hv[tid];
+      a.cnt[j] = hc[tid];
+      if (hc[tid] > a.cap) atomicAdd(a.nover, 1);
+    }
+  }
+}
+
+// Pass 2: one wave per query.  LDS heap per wave: k doubles + k ints.
+template <typename F>
+__device__ void wave_rerank_each(const Bf16Args &a, int j, F &&visit);
+
+__device__ __forceinline__ double wave_sum(double v) {
+  // Fixed tree: lane 0 ends with the sum, then broadcast; the same bits on
+  // every call for the same inputs.
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
+__device__ void heap_sift_d(double *hd, int *hi, int s, int k, double xd, int xi) {
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    double cd = hd[c];
+    int ci = hi[c];
+    if (c + 1 < k && cless_d(cd, ci, hd[c + 1], hi[c + 1])) {
+      ++c;
+      cd = hd[c];
+      ci = hi[c];
+    }
+    if (!cless_d(xd, xi, cd, ci)) break;
+    hd[s] = cd;
+    hi[s] = ci;
+    s = c;
+  }
+  hd[s] = xd;
+  hi[s] = xi;
+}
+
+__global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  double *hd = (double *)smem + (size_t)w * k;
+  int *hi = (int *)(smem + al16r(sizeof(double) * 4 * (size_t)k)) + (size_t)w * k;
+  if (j >= a.n) return;  // whole wave leaves together; no block barrier below
+  const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  if (lane == 0) {
+    for (int t = 0; t < k; ++t) {
+      hd[t] = a.D[row + t];
+      hi[t] = a.I[row + t];
+    }
+    for (int s = k / 2 - 1; s >= 0; --s) heap_sift_d(hd, hi, s, k, hd[s], hi[s]);
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const float *xq = a.XB + (size_t)a.bmap[j] * a.d;
+  const float Uf = a.U[j];
+  const float Ulim = Uf * (1.0f + 0x1p-20f) + 1e-30f;
+  const int cnt = a.cnt[j];
+  const bool over = cnt > a.cap;
+  const int nscan = over ? a.m : cnt;
+  double rd = hd[0];
+  int ri = hi[0];
+  for (int e = 0; e < nscan; ++e) {
+    int pos;
+    if (over) {
+      pos = e;
+    } else {
+      const int2 en = a.lst[(size_t)j * a.cap + e];
+      if (__int_as_float(en.y) > Ulim) continue;
+      pos = en.x;
+    }
+    const int id = a.amap[pos];
+    const float *xa = a.XA + (size_t)id * a.d;
+    double s = 0.0;
+    for (int p = lane; p < a.d; p += 64) {
+      const double t = (double)xa[p] - (double)xq[p];
+      s = __builtin_fma(t, t, s);
+    }
+    const double dist = wave_sum(s);
+    if (cless_d(dist, id, rd, ri)) {
+      if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, id);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      rd = hd[0];
+      ri = hi[0];
+    }
+  }
+  if (lane == 0) {
+    for (int e2 = k - 1; e2 > 0; --e2) {
+      const double xd = hd[e2];
+      const int xi = hi[e2];
+      hd[e2] = hd[0];
+      hi[e2] = hi[0];
+      heap_sift_d(hd, hi, 0, e2, xd, xi);
+    }
+    for (int t = 0; t < k; ++t) {
+      a.D[row + t] = hd[k - 1 - t];
+      a.I[row + t] = hi[k - 1 - t];
+    }
+  }
+}
+
+namespace {
+
+constexpr size_t kLds = 160 * 1024;
+constexpr int kCBb = 4;
+
+struct Bf16Plan {
+  int ks = 0, qb = 0, w = 0, q = 0, cap = 0;
+  size_t lds = 0;
+};
+
+Bf16Plan plan_bf16(int d, int k) {
+  Bf16Plan p;
+  const int need = (d + 31) / 32;
+  for (int ks : {1, 2, 4, 8}) {
+    if (need <= ks) {
+      p.ks = ks;
+      break;
+    }
+  }
+  if (p.ks == 0) return p;
+  const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
+  for (auto &o : opts) {
+    const int Q = o[1] * 16 * o[0];
+    const size_t b = FilterLayout(p.ks, kCBb, Q, k).bytes;
+    if (b <= kLds) {
+      p.qb = o[0];
+      p.w = o[1];
+      p.q = Q;
+      p.lds = b;
+      break;
+    }
+  }
+  int cap = 16 * k;
+  p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
+  return p;
+}
+
+size_t bf16_work_layout(int n, int cap, size_t *o_lst, size_t *o_cnt, size_t *o_u, size_t *o_over) {
+  size_t off = 0;
+  *o_lst = off;
+  off = al16r(off + (size_t)n * cap * sizeof(int2));
+  *o_cnt = off;
+  off = al16r(off + (size_t)n * 4);
+  *o_u = off;
+  off = al16r(off + (size_t)n * 4);
+  *o_over = off;
+  off = al16r(off + 16);
+  return off;
+}
+
+template <int KS>
+hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s) {
+  const dim3 grid((a.n + p.q - 1) / p.q);
+#define GSKNN_BF16_CASE(QB_, W_)                                                       \
+  if (p.qb == QB_ && p.w == W_) {                                                      \
+    auto kern = knn_bf16_filter_kernel<KS, QB_, kCBb, W_>;                              \
+    hipError_t e = hipFuncSetAttribute((const void *)kern,                              \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                       (int)p.lds);                                     \
+    if (e != hipSuccess) return e;                                                      \
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);                         \
+    return hipGetLastError();                                                           \
+  }
+  GSKNN_BF16_CASE(2, 4)
+  GSKNN_BF16_CASE(1, 4)
+  GSKNN_BF16_CASE(1, 2)
+  GSKNN_BF16_CASE(1, 1)
+#undef GSKNN_BF16_CASE
+  return hipErrorNotSupported;
+}
+
+}  // namespace
+
+size_t bf16_work_bytes(int m, int n, int d, int k) {
+  (void)m;
+  const Bf16Plan p = plan_bf16(d, k);
+  if (p.ks == 0 || p.q == 0 || n <= 0) return 0;
+  size_t a, b, c, e;
+  return bf16_work_layout(n, p.cap, &a, &b, &c, &e);
+}
+
+hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const float *XA,
+                           const void *prepA, const int *amap, const float *XB, const void *prepB,
+                           const int *bmap, double *D, int *I, int ldk, const int *hrow,
+                           void *work, size_t work_bytes, int *overflow_out, hipStream_t s) {
+  if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return hipSuccess;
+  const Bf16Plan p = plan_bf16(d, k);
+  if (p.ks == 0 || p.q == 0) return hipErrorNotSupported;
+  const size_t rl = al16r(sizeof(double) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;
+  if (rl > kLds) return hipErrorNotSupported;
+  size_t o_lst, o_cnt, o_u, o_over;
+  const size_t need = bf16_work_layout(n, p.cap, &o_lst, &o_cnt, &o_u, &o_over);
+  if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
+  Bf16Args a{};
+  a.m = m; a.n = n; a.d = d; a.k = k; a.ldk = ldk; a.cap = p.cap;
+  a.NA = NA; a.NB = NB;
+  a.XA = XA; a.XB = XB;
+  a.prepA = (const unsigned char *)prepA;
+  a.prepB = (const unsigned char *)prepB;
+  a.amap = amap; a.bmap = bmap;
+  a.D = D; a.I = I; a.hrow = hrow;
+  a.lst = (int2 *)((char *)work + o_lst);
+  a.cnt = (int *)((char *)work + o_cnt);
+  a.U = (float *)((char *)work + o_u);
+  a.nover = (int *)((char *)work + o_over);
+  hipError_t e = hipMemsetAsync(a.nover, 0, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  switch (p.ks) {
+    case 1: e = launch_filter_ks<1>(p, a, s); break;
+    case 2: e = launch_filter_ks<2>(p, a, s); break;
+    case 4: e = launch_filter_ks<4>(p, a, s); break;
+    case 8: e = launch_filter_ks<8>(p, a, s); break;
+    default: return hipErrorNotSupported;
+  }
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)knn_rerank_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)rl);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(knn_rerank_kernel, dim3((n + 3) / 4), dim3(256), rl, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (overflow_out)
+    e = hipMemcpyAsync(overflow_out, a.nover, sizeof(int), hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
+}  // namespace gsknn

@@ -75,6 +75,12 @@ def _declare(L):
     L.gsknn_dknn_work_bytes.restype = sz
     L.gsknn_sknn_work_bytes.argtypes = [i32] * 4
     L.gsknn_sknn_work_bytes.restype = sz
+    L.gsknn_bf16_prep_bytes.argtypes = [i32, i32]
+    L.gsknn_bf16_prep_bytes.restype = sz
+    L.gsknn_bf16_prep_dev.argtypes = [i32, i32, vp, vp, vp]
+    L.gsknn_bf16_work_bytes.argtypes = [i32] * 4
+    L.gsknn_bf16_work_bytes.restype = sz
+    L.gsknn_bf16_knn_dev.argtypes = [i32] * 5 + [vp] * 3 + [i32] + [vp] * 5 + [i32, vp, vp, sz, vp, vp]
     L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     for name in ("dgsknn_var1", "sgsknn_var1"):
@@ -235,5 +241,39 @@ def work_bytes_s(m, n, d, k) -> int:
 def sqnorm_dev(X, X2, stream=None):
     fn = lib().gsknn_dsqnorm_dev if X.element_size() == 8 else lib().gsknn_ssqnorm_dev
     rc = fn(X.shape[0], X.shape[1], _dp(X), _dp(X2), _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+# ---------------------------------------------------------------------------
+# BF16 candidate pass + FP64 re-rank (device tensors).
+# ---------------------------------------------------------------------------
+def bf16_prep_dev(X, stream=None):
+    """Prepare an FP32 device table (N, d) for the BF16 path; returns the
+    prepared buffer (uint8 device tensor)."""
+    import torch
+    N, d = X.shape
+    nb = int(lib().gsknn_bf16_prep_bytes(N, d))
+    prep = torch.empty(max(nb, 16), dtype=torch.uint8, device=X.device)
+    rc = lib().gsknn_bf16_prep_dev(N, d, _dp(X), _dp(prep), _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+    return prep
+
+
+def bf16_work_bytes(m, n, d, k) -> int:
+    return int(lib().gsknn_bf16_work_bytes(m, n, d, k))
+
+
+def bf16_knn_dev(XA, prepA, amap, XB, prepB, bmap, D, I, hrow=None, work=None, overflow=None,
+                 stream=None):
+    """Exact kNN (FP64 distances) of queries XB[bmap] among XA[amap] from a
+    BF16 candidate pass; D (float64) / I (int32) rows updated in place."""
+    m, n, d, k = amap.numel(), bmap.numel(), XA.shape[1], D.shape[1]
+    wb = work.numel() * work.element_size() if work is not None else 0
+    rc = lib().gsknn_bf16_knn_dev(m, n, d, k, XA.shape[0], _dp(XA), _dp(prepA), _dp(amap),
+                                  XB.shape[0], _dp(XB), _dp(prepB), _dp(bmap), _dp(D), _dp(I),
+                                  D.stride(0), _dp(hrow), _dp(work), wb, _dp(overflow),
+                                  _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())

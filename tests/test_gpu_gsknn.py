@@ -285,3 +285,62 @@ def test_deterministic_and_empty():
     D3, I3 = D1.copy(), I1.copy()
     gsknn.dgsknn_heap(p.XA, p.XA2, p.amap[:0].copy(), p.XB, p.XB2, p.bmap, D3, I3)
     assert np.array_equal(D1, D3) and np.array_equal(I1, I3)
+
+
+def _bf16_run(XA32, amap, XB32, bmap, D, I, hrow=None):
+    import torch
+    dev = torch.device("cuda")
+    tA = torch.from_numpy(XA32).to(dev)
+    tB = tA if XB32 is XA32 else torch.from_numpy(XB32).to(dev)
+    pA = gsknn.bf16_prep_dev(tA)
+    pB = pA if XB32 is XA32 else gsknn.bf16_prep_dev(tB)
+    tD, tI = torch.from_numpy(D).to(dev), torch.from_numpy(I).to(dev)
+    over = torch.zeros(1, dtype=torch.int32, device=dev)
+    gsknn.bf16_knn_dev(tA, pA, torch.from_numpy(amap).to(dev), tB, pB,
+                       torch.from_numpy(bmap).to(dev), tD, tI, overflow=over)
+    torch.cuda.synchronize()
+    return tD.cpu().numpy(), tI.cpu().numpy(), int(over.item())
+
+
+@pytest.mark.parametrize("d,k", [(256, 64), (128, 32), (32, 16), (50, 10)])
+def test_bf16_rerank_matches_fp64_oracle(d, k):
+    # Low-rank generator of the precision workload (latent 10, noise 0.01).
+    g = np.random.Generator(np.random.PCG64(d + k))
+    N = 6000
+    X = ds.low_rank(N, d, 10, seed=d)
+    perm = g.permutation(N).astype(np.int32)
+    amap, bmap = perm[:4000].copy(), perm[4000:].copy()
+    D, I = ds.empty_heap(bmap.size, k)
+    print(f"[gpu] bf16 d={d} k={k}", flush=True)
+    Dg, Ig, over = _bf16_run(X, amap, X, bmap, D.copy(), I.copy())
+    # Oracle: dgsknn_ref arithmetic on the FP64-upcast points.
+    X64 = X.astype(np.float64)
+    X2 = oracle.sqnorm(X64)
+    Dr, Ir = D.copy(), I.copy()
+    oracle.gsknn(X64, X2, amap, X64, X2, bmap, Dr, Ir)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+    # and the FP64 direct-difference definition of the re-rank
+    Dd, Id = D.copy(), I.copy()
+    oracle.direct_knn_f32in(X, amap, X, bmap, Dd, Id)
+    assert_rows(Dg, Ig, Dd, Id, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+    print(f"[gpu] bf16 overflowed queries: {over}", flush=True)
+
+
+def test_bf16_carry_over_and_ties():
+    # Integer points (exact in BF16) with many exact ties, two calls into the
+    # same heap rows: canonical (distance, lower index) result.
+    g = np.random.Generator(np.random.PCG64(77))
+    X = np.floor(g.random((3000, 40)) * 4).astype(np.float32)
+    amap1 = g.integers(0, 3000, 1500).astype(np.int32)
+    amap2 = g.integers(0, 3000, 1700).astype(np.int32)
+    bmap = g.permutation(3000)[:500].astype(np.int32)
+    D, I = ds.empty_heap(500, 20)
+    Dg, Ig, _ = _bf16_run(X, amap1, X, bmap, D, I)
+    Dg, Ig, _ = _bf16_run(X, amap2, X, bmap, Dg.copy(), Ig.copy())
+    X64 = X.astype(np.float64)
+    X2 = oracle.sqnorm(X64)
+    Dr, Ir = ds.empty_heap(500, 20)
+    for am in (amap1, amap2):
+        oracle.gsknn(X64, X2, am, X64, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(Dg, Ig, Dr, Ir)
