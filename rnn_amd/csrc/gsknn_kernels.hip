@@ -266,19 +266,27 @@ struct MfmaLayout {
     const size_t tile = (size_t)KS * CB * 64;
     const size_t C = 16 * (size_t)CB;
     tA = 0;
-    tn = al16(tA + sizeof(double) * 2 * tile);
-    ti = al16(tn + sizeof(double) * 2 * C);
-    hd = al16(ti + sizeof(int) * 2 * C);
+    tn = al16(tA + sizeof(double) * tile);
+    ti = al16(tn + sizeof(double) * C);
+    hd = al16(ti + sizeof(int) * C);
     hi = al16(hd + sizeof(double) * (size_t)k * Q);
     bytes = al16(hi + sizeof(int) * (size_t)k * Q);
   }
 };
 
+// Candidate tiles go global -> registers -> LDS: thread t stages dims
+// [p0, p0 + DPT) of candidate slot t / TPC.  The amap entries are read two
+// tiles ahead and the point rows one tile ahead, so no load waits on another
+// inside the loop and the loads overlap the MFMA work of the current tile.
+// One LDS tile buffer (two barriers per tile) keeps the C=32, k=32 workgroup
+// at 64 KB so that two workgroups share a CU.
 template <int KS, int QB, int CB, int W>
 __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a) {
   constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, DP = 4 * KS, NT = 64 * W;
-  constexpr int TILE = KS * CB * 64;
-  constexpr int LPT = (C * DP + NT - 1) / NT;
+  static_assert(NT % C == 0, "threads per candidate");
+  constexpr int TPC = NT / C;
+  static_assert(DP % TPC == 0, "dims per thread");
+  constexpr int DPT = DP / TPC;
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k, d = a.d;
   const MfmaLayout L(KS, CB, Q, k);
@@ -307,53 +315,59 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
     ql[qb] = w * QW + qb * 16 + (lane & 15);
     const int j = qbase + ql[qb];
     qv[qb] = j < a.n;
-    const double *xq = a.XB + (size_t)(qv[qb] ? a.bmap[j] : 0) * d;
-    qn[qb] = qv[qb] ? a.XB2[a.bmap[j]] : 0.0;
+    const int jj = qv[qb] ? j : a.n - 1;
+    const double *xq = a.XB + (size_t)a.bmap[jj] * d;
+    qn[qb] = qv[qb] ? a.XB2[a.bmap[jj]] : 0.0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       const int p = 4 * s + grp;
-      bq[qb][s] = (qv[qb] && p < d) ? xq[p] : 0.0;
+      const double v = xq[p < d ? p : d - 1];
+      bq[qb][s] = (qv[qb] && p < d) ? v : 0.0;
     }
   }
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
-  double stage[LPT];
-  auto load_tile = [&](int t) {
-    const int c0 = cbeg + t * C;
-    const int cnt = min(C, cend - c0);
-#pragma unroll
-    for (int u = 0; u < LPT; ++u) {
-      const int e = tid + u * NT;
-      double v = 0.0;
-      if (e < C * DP) {
-        const int c = e / DP, p = e - (e / DP) * DP;
-        if (c < cnt && p < d) v = a.XA[(size_t)a.amap[c0 + c] * d + p];
-      }
-      stage[u] = v;
-    }
+  const int cs = tid / TPC;
+  const int p0 = (tid % TPC) * DPT;
+  double stage[DPT];
+  double stage_n = 0.0;
+  auto cand_id = [&](int t) -> int {
+    const int c = min(cbeg + t * C + cs, cend - 1);
+    return a.amap[c];
   };
-  auto store_tile = [&](int t, int buf) {
-    const int c0 = cbeg + t * C;
-    const int cnt = min(C, cend - c0);
-    double *A = tA + buf * TILE;
+  auto load_x = [&](int id) {
+    const double *row = a.XA + (size_t)id * d;
 #pragma unroll
-    for (int u = 0; u < LPT; ++u) {
-      const int e = tid + u * NT;
-      if (e < C * DP) {
-        const int c = e / DP, p = e - (e / DP) * DP;
-        const int s = p >> 2, kk = p & 3, cb = c >> 4, i = c & 15;
-        A[(s * CB + cb) * 64 + i + 16 * kk] = stage[u];
-      }
+    for (int e = 0; e < DPT; ++e) {
+      const int p = p0 + e;
+      stage[e] = row[p < d ? p : d - 1];
     }
-    if (tid < C) {
-      const bool ok = tid < cnt;
-      const int id = ok ? a.amap[c0 + tid] : INT_MAX;
-      ti[buf * C + tid] = id;
-      tn[buf * C + tid] = ok ? a.XA2[id] : 0.0;
+    stage_n = a.XA2[id];
+  };
+  auto store_x = [&](int t, int id) {
+    const bool ok = cbeg + t * C + cs < cend;
+    const int cb = cs >> 4, i = cs & 15;
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) {
+      const int p = p0 + e;
+      const int s = p >> 2, kk = p & 3;
+      tA[(s * CB + cb) * 64 + i + 16 * kk] = (ok && p < d) ? stage[e] : 0.0;
+    }
+    if (tid % TPC == 0) {
+      tn[cs] = ok ? stage_n : 0.0;
+      ti[cs] = ok ? id : INT_MAX;
     }
   };
 
-  __syncthreads();  // heaps loaded
+  int id_n = 0;
+  if (ntiles > 0) {
+    const int id0 = cand_id(0);
+    load_x(id0);
+    id_n = ntiles > 1 ? cand_id(1) : id0;
+    __syncthreads();  // heaps loaded
+    store_x(0, id0);
+  }
+  __syncthreads();
   double rd[QB];
   int ri[QB];
 #pragma unroll
@@ -362,38 +376,28 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
     ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
   }
 
-  if (ntiles > 0) {
-    load_tile(0);
-    store_tile(0, 0);
-  }
-  __syncthreads();
-
   for (int t = 0; t < ntiles; ++t) {
-    const int buf = t & 1;
-    if (t + 1 < ntiles) load_tile(t + 1);
+    const bool more = t + 1 < ntiles;
+    const int id_nn = (t + 2 < ntiles) ? cand_id(t + 2) : id_n;
+    if (more) load_x(id_n);
 
     dbl4 acc[QB][CB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-    const double *A = tA + buf * TILE;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
-        const double av = A[(s * CB + cb) * 64 + lane];
+        const double av = tA[(s * CB + cb) * 64 + lane];
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
 
-    const int c0 = cbeg + t * C;
-    const int cnt = min(C, cend - c0);
-    const double *TN = tn + buf * C;
-    const int *TI = ti + buf * C;
+    const int cnt = min(C, cend - (cbeg + t * C));
     unsigned pass[QB];
     bool anyp = false;
 #pragma unroll
@@ -404,9 +408,10 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int c = cb * 16 + grp + 4 * v;
-          const double dist = xadd(xadd(xmul(acc[qb][cb][v], -2.0), TN[c]), qn[qb]);
+          // (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]]; -2 * dot is exact.
+          const double dist = xadd(__builtin_fma(acc[qb][cb][v], -2.0, tn[c]), qn[qb]);
           acc[qb][cb][v] = dist;
-          if (qv[qb] && c < cnt && cless(dist, TI[c], rd[qb], ri[qb])) {
+          if (qv[qb] && c < cnt && cless(dist, ti[c], rd[qb], ri[qb])) {
             pass[qb] |= 1u << (cb * 4 + v);
             anyp = true;
           }
@@ -428,7 +433,7 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
                 if (pass[qb] & (1u << (cb * 4 + v))) {
                   const int c = cb * 16 + grp + 4 * v;
                   const double dist = acc[qb][cb][v];
-                  const int idx = TI[c];
+                  const int idx = ti[c];
                   if (cless(dist, idx, r0, r1)) {
                     sift_down(hd, hi, Q, ql[qb], 0, k, dist, idx);
                     r0 = hd[ql[qb]];
@@ -447,8 +452,10 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
         ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
       }
     }
-    if (t + 1 < ntiles) store_tile(t + 1, buf ^ 1);
-    __syncthreads();
+    __syncthreads();  // tile consumed
+    if (more) store_x(t + 1, id_n);
+    __syncthreads();  // next tile visible
+    id_n = id_nn;
   }
 
   if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
