@@ -354,7 +354,9 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
       tA[(s * CB + cb) * 64 + i + 16 * kk] = (ok && p < d) ? stage[e] : 0.0;
     }
     if (tid % TPC == 0) {
-      tn[cs] = ok ? stage_n : 0.0;
+      // An empty slot gets distance +inf and index INT_MAX: it loses to
+      // every row entry, including (+inf, -1) placeholders.
+      tn[cs] = ok ? stage_n : __builtin_inf();
       ti[cs] = ok ? id : INT_MAX;
     }
   };
@@ -369,12 +371,8 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
   }
   __syncthreads();
   double rd[QB];
-  int ri[QB];
 #pragma unroll
-  for (int qb = 0; qb < QB; ++qb) {
-    rd[qb] = qv[qb] ? hd[ql[qb]] : 0.0;
-    ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
-  }
+  for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
 
   for (int t = 0; t < ntiles; ++t) {
     const bool more = t + 1 < ntiles;
@@ -397,7 +395,8 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
       }
     }
 
-    const int cnt = min(C, cend - (cbeg + t * C));
+    // Hot path: one FP64 compare per pair against the row's current root;
+    // the exact (distance, index) test runs only for the rare survivors.
     unsigned pass[QB];
     bool anyp = false;
 #pragma unroll
@@ -411,46 +410,40 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
           // (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]]; -2 * dot is exact.
           const double dist = xadd(__builtin_fma(acc[qb][cb][v], -2.0, tn[c]), qn[qb]);
           acc[qb][cb][v] = dist;
-          if (qv[qb] && c < cnt && cless(dist, ti[c], rd[qb], ri[qb])) {
+          if (dist <= rd[qb]) {
             pass[qb] |= 1u << (cb * 4 + v);
             anyp = true;
           }
         }
     }
     if (__any(anyp)) {
-#pragma unroll
+      // The four lane groups cover the wave's queries once each, so within a
+      // group every lane owns a distinct heap; groups take turns.
+#pragma unroll 1
       for (int g = 0; g < 4; ++g) {
         if (grp == g) {
 #pragma unroll
           for (int qb = 0; qb < QB; ++qb) {
-            if (!pass[qb]) continue;
-            double r0 = hd[ql[qb]];
-            int r1 = hi[ql[qb]];
+            unsigned bits = pass[qb];
+            const int slot = ql[qb];
+            while (bits) {
+              const int b = __builtin_ctz(bits);
+              bits &= bits - 1;
+              double dist = acc[qb][0][0];
 #pragma unroll
-            for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                if (pass[qb] & (1u << (cb * 4 + v))) {
-                  const int c = cb * 16 + grp + 4 * v;
-                  const double dist = acc[qb][cb][v];
-                  const int idx = ti[c];
-                  if (cless(dist, idx, r0, r1)) {
-                    sift_down(hd, hi, Q, ql[qb], 0, k, dist, idx);
-                    r0 = hd[ql[qb]];
-                    r1 = hi[ql[qb]];
-                  }
-                }
-              }
+              for (int bb = 1; bb < CB * 4; ++bb)
+                if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
+              const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
+              const int idx = ti[c];
+              if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+            }
           }
         }
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
 #pragma unroll
-      for (int qb = 0; qb < QB; ++qb) {
-        rd[qb] = qv[qb] ? hd[ql[qb]] : 0.0;
-        ri[qb] = qv[qb] ? hi[ql[qb]] : 0;
-      }
+      for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
     }
     __syncthreads();  // tile consumed
     if (more) store_x(t + 1, id_n);
