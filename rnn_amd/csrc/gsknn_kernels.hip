@@ -374,6 +374,58 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
 
+  constexpr int QR = 4;
+  double pq_d[QR];
+  int pq_i[QR], pq_q[QR];
+  int pq_n = 0;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+    pq_d[r] = 0.0;
+    pq_i[r] = 0;
+    pq_q[r] = 0;
+  }
+  // Apply the queued survivors and this tile's (pass bits over acc) to the
+  // heaps.  The four lane groups cover the wave's queries once each, so
+  // within a group every lane owns a distinct heap; groups take turns.
+  auto flush = [&](const unsigned *pass, const dbl4 (*acc)[CB]) {
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+      if (grp == g) {
+#pragma unroll
+        for (int r = 0; r < QR; ++r) {
+          if (r < pq_n) {
+            int slot = ql[0];
+#pragma unroll
+            for (int qb = 1; qb < QB; ++qb)
+              if (pq_q[r] == qb) slot = ql[qb];
+            if (cless(pq_d[r], pq_i[r], hd[slot], hi[slot]))
+              sift_down(hd, hi, Q, slot, 0, k, pq_d[r], pq_i[r]);
+          }
+        }
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass[qb];
+          const int slot = ql[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            double dist = acc[qb][0][0];
+#pragma unroll
+            for (int bb = 1; bb < CB * 4; ++bb)
+              if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
+            const int idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+            if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+    pq_n = 0;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
+  };
+
   for (int t = 0; t < ntiles; ++t) {
     const bool more = t + 1 < ntiles;
     const int id_nn = (t + 2 < ntiles) ? cand_id(t + 2) : id_n;
@@ -416,34 +468,38 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
           }
         }
     }
-    if (__any(anyp)) {
-      // The four lane groups cover the wave's queries once each, so within a
-      // group every lane owns a distinct heap; groups take turns.
-#pragma unroll 1
-      for (int g = 0; g < 4; ++g) {
-        if (grp == g) {
+    // Survivors wait in a per-lane queue of QR entries; the heaps (and the
+    // roots used by the filter) are updated only when some lane's queue
+    // would overflow, which keeps the slow path off most tiles.  A stale
+    // root only lets more pairs through the filter; every queued pair is
+    // tested exactly against the live root when it is applied.
+    int npass = 0;
 #pragma unroll
-          for (int qb = 0; qb < QB; ++qb) {
-            unsigned bits = pass[qb];
-            const int slot = ql[qb];
-            while (bits) {
-              const int b = __builtin_ctz(bits);
-              bits &= bits - 1;
-              double dist = acc[qb][0][0];
+    for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
+    if (__any(pq_n + npass > QR)) {
+      flush(pass, acc);
+    } else if (__any(anyp)) {
 #pragma unroll
-              for (int bb = 1; bb < CB * 4; ++bb)
-                if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
-              const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
-              const int idx = ti[c];
-              if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+      for (int qb = 0; qb < QB; ++qb) {
+        unsigned bits = pass[qb];
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1;
+          double dist = acc[qb][0][0];
+#pragma unroll
+          for (int bb = 1; bb < CB * 4; ++bb)
+            if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
+          const int idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+#pragma unroll
+          for (int r = 0; r < QR; ++r)
+            if (pq_n == r) {
+              pq_d[r] = dist;
+              pq_i[r] = idx;
+              pq_q[r] = qb;
             }
-          }
+          ++pq_n;
         }
-        __builtin_amdgcn_wave_barrier();
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
     }
     __syncthreads();  // tile consumed
     if (more) store_x(t + 1, id_n);
@@ -451,6 +507,12 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
     id_n = id_nn;
   }
 
+  {
+    const unsigned none[QB] = {};
+    const dbl4 zero[QB][CB] = {};
+    if (__any(pq_n > 0)) flush(none, zero);
+  }
+  __syncthreads();
   if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
 }
 
