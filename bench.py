@@ -57,12 +57,45 @@ def cpu_baseline(p, seconds_target=12.0):
                       f"{dt:.2f} s, oracle/gsknn_oracle.c -O3 -fopenmp"}
 
 
+def copy_calibration(torch, dev, args):
+    """Plain device-to-device copy of a known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: each launch reads and writes exactly
+    COPY_BYTES."""
+    nbytes = COPY_BYTES
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"workload": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "ms": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
+
+
+COPY_BYTES = 1 << 30
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "knn_mfma_f64_config2_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "copy"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma"])
     args = ap.parse_args()
@@ -78,6 +111,9 @@ def main():
     from rnn_amd import gsknn
 
     dev = torch.device("cuda:0")
+    if args.workload == "copy":
+        copy_calibration(torch, dev, args)
+        return
     t0 = time.perf_counter()
     p = ds.config2()
     log(f"data N={p.XA.shape[0]} d={p.d} m={p.m} n={p.n} k={p.k} ({time.perf_counter() - t0:.1f}s)")
@@ -166,7 +202,7 @@ def main():
                    "m": p.m, "n": p.n, "d": p.d, "k": p.k, "mode": args.mode},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
                      "peak": FP64_MATRIX_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_MATRIX_PEAK, "traffic": None,
+                     "frac": achieved / FP64_MATRIX_PEAK, "traffic": measured_traffic(),
                      "kernel_ms": kern_ms, "flop_per_launch": flops},
         "spot_parity": parity,
     }
