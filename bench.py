@@ -90,12 +90,129 @@ def measured_traffic():
         return None
 
 
+def _timed(torch, stream, fn, reps):
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return sum(ts) / len(ts), ts
+
+
+def precision_form(torch, dev, args):
+    """Config 3: FP32 points x = zW + 0.01 eps (latent 10), d = 256,
+    m = n = 131072, k = 64.  Times the BF16-candidate + FP64 re-rank form,
+    the plain FP64 MFMA path (points upcast) and the plain FP32 path, each on
+    empty heaps, and compares neighbour indices and distances."""
+    import numpy as np
+
+    from rnn_amd import datasets as ds
+    from rnn_amd import gsknn
+
+    t0 = time.perf_counter()
+    p = ds.config3(scale=args.scale)
+    log(f"c3 data N={p.XA.shape[0]} d={p.d} m={p.m} n={p.n} k={p.k} ({time.perf_counter() - t0:.1f}s)")
+    stream = torch.cuda.current_stream()
+    X32 = torch.from_numpy(p.XA).to(dev)
+    X2_32 = torch.from_numpy(p.XA2).to(dev)
+    X64 = X32.double()
+    X2_64 = torch.empty(X64.shape[0], dtype=torch.float64, device=dev)
+    gsknn.sqnorm_dev(X64, X2_64)
+    am = torch.from_numpy(p.amap).to(dev)
+    bm = torch.from_numpy(p.bmap).to(dev)
+    pairs = float(p.m) * p.n
+    out = {"workload": "c3", "m": p.m, "n": p.n, "d": p.d, "k": p.k}
+
+    # BF16 candidates + FP64 re-rank.
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    prep = gsknn.bf16_prep_dev(X32)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out["bf16_prep_ms"] = e0.elapsed_time(e1)
+    wb = gsknn.bf16_work_bytes(p.m, p.n, p.d, p.k)
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    over = torch.zeros(1, dtype=torch.int32, device=dev)
+    Db = torch.empty((p.n, p.k), dtype=torch.float64, device=dev)
+    Ib = torch.empty((p.n, p.k), dtype=torch.int32, device=dev)
+
+    def bf16_call():
+        Db.fill_(float("inf"))
+        Ib.fill_(-1)
+        gsknn.bf16_knn_dev(X32, prep, am, X32, prep, bm, Db, Ib, work=work, overflow=over,
+                           stream=stream)
+
+    bf16_call()
+    torch.cuda.synchronize()
+    ms_b, _ = _timed(torch, stream, bf16_call, max(1, args.steps))
+    out["bf16_rerank"] = {"ms": ms_b, "pairs_per_s": pairs / (ms_b * 1e-3),
+                          "overflowed_queries": int(over.item()),
+                          "bf16_tflops_equiv": 2.0 * p.d * pairs / (ms_b * 1e-3) / 1e12}
+    log(f"bf16+rerank {ms_b:.2f} ms, overflow {int(over.item())}")
+
+    # Plain FP64 path (MFMA).
+    D64 = torch.empty((p.n, p.k), dtype=torch.float64, device=dev)
+    I64 = torch.empty((p.n, p.k), dtype=torch.int32, device=dev)
+    w64 = torch.empty(max(16, gsknn.work_bytes_d(p.m, p.n, p.d, p.k)), dtype=torch.uint8, device=dev)
+
+    def f64_call():
+        D64.fill_(float("inf"))
+        I64.fill_(-1)
+        gsknn.dknn_dev(X64, X2_64, am, X64, X2_64, bm, D64, I64, work=w64, stream=stream)
+
+    ms_64, _ = _timed(torch, stream, f64_call, 1)
+    out["fp64"] = {"ms": ms_64, "pairs_per_s": pairs / (ms_64 * 1e-3),
+                   "tflops": 2.0 * p.d * pairs / (ms_64 * 1e-3) / 1e12,
+                   "frac_fp64_peak": 2.0 * p.d * pairs / (ms_64 * 1e-3) / FP64_MATRIX_PEAK}
+    log(f"fp64 {ms_64:.2f} ms")
+
+    # Plain FP32 path (reference-oracle arithmetic).
+    D32 = torch.empty((p.n, p.k), dtype=torch.float32, device=dev)
+    I32 = torch.empty((p.n, p.k), dtype=torch.int32, device=dev)
+    w32 = torch.empty(max(16, gsknn.work_bytes_s(p.m, p.n, p.d, p.k)), dtype=torch.uint8, device=dev)
+
+    def f32_call():
+        D32.fill_(float("inf"))
+        I32.fill_(-1)
+        gsknn.sknn_dev(X32, X2_32, am, X32, X2_32, bm, D32, I32, work=w32, stream=stream)
+
+    ms_32, _ = _timed(torch, stream, f32_call, 1)
+    out["fp32"] = {"ms": ms_32, "pairs_per_s": pairs / (ms_32 * 1e-3)}
+    log(f"fp32 {ms_32:.2f} ms")
+    out["speedup_bf16_vs_fp64"] = ms_64 / ms_b
+
+    from oracle import oracle
+    Ibs = oracle.sort_rows(Db.cpu().numpy(), Ib.cpu().numpy())
+    I64s = oracle.sort_rows(D64.cpu().numpy(), I64.cpu().numpy())
+    out["rows_equal_bf16_vs_fp64_gpu"] = float(np.mean(np.all(Ibs[1] == I64s[1], axis=1)))
+    # FP64 oracle (dgsknn_ref arithmetic on the upcast points) on a row sample.
+    rows = np.linspace(0, p.n - 1, 64).astype(np.int64)
+    Xu = p.XA.astype(np.float64)
+    X2u = oracle.sqnorm(Xu)
+    Dr = np.full((rows.size, p.k), np.inf)
+    Ir = np.full((rows.size, p.k), -1, dtype=np.int32)
+    oracle.gsknn(Xu, X2u, p.amap, Xu, X2u, p.bmap[rows].copy(), Dr, Ir)
+    Drs, Irs = oracle.sort_rows(Dr, Ir)
+    out["sample_rows"] = int(rows.size)
+    out["sample_index_match_bf16"] = bool(np.array_equal(Ibs[1][rows], Irs))
+    out["sample_index_match_fp64"] = bool(np.array_equal(I64s[1][rows], Irs))
+    rel = np.abs(Ibs[0][rows] - Drs) / np.maximum(np.abs(Drs), 1e-300)
+    out["bf16_max_rel_dist_err_vs_fp64_oracle"] = float(rel.max())
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "copy"])
+    ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma"])
     args = ap.parse_args()
@@ -113,6 +230,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "copy":
         copy_calibration(torch, dev, args)
+        return
+    if args.workload == "c3":
+        precision_form(torch, dev, args)
         return
     t0 = time.perf_counter()
     p = ds.config2()

@@ -639,7 +639,9 @@ int mfma_ks(int d) {
   return 0;
 }
 
-constexpr int kCB = 2;
+// Candidate blocks of 16 per tile: two for d <= 64, one above, where the
+// query fragments already take most of the registers.
+constexpr int mfma_cb(int ks) { return ks >= 32 ? 1 : 2; }
 
 size_t merge_lds(int k, int elem, int qpb) {
   return al16((size_t)elem * k * qpb) + sizeof(int) * (size_t)k * qpb;
@@ -663,7 +665,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
     for (auto &o : opts) {
       const int Q = o[1] * 16 * o[0];
-      const size_t b = MfmaLayout(ks, kCB, Q, k).bytes;
+      const size_t b = MfmaLayout(ks, mfma_cb(ks), Q, k).bytes;
       if (b <= kLdsMax) {
         p.qb = o[0];
         p.w = o[1];
@@ -673,7 +675,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       }
     }
     if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
-    tile_c = 16 * kCB;
+    tile_c = 16 * mfma_cb(ks);
   }
   if (p.kind == 0) {
     p.ct = exact_ct(d, elem);
@@ -726,7 +728,7 @@ hipError_t launch_mfma_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s
   hipError_t e = hipSuccess;
 #define GSKNN_MFMA_CASE(QB_, W_)                                              \
   if (p.qb == QB_ && p.w == W_) {                                             \
-    auto kern = knn_mfma_f64_kernel<KS, QB_, kCB, W_>;                        \
+    auto kern = knn_mfma_f64_kernel<KS, QB_, mfma_cb(KS), W_>;                \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
     return hipGetLastError();                                                 \
