@@ -138,16 +138,15 @@ struct Bf16Args {
 __device__ __forceinline__ int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
 
 struct FilterLayout {
-  size_t tA, nc, xn, en, pos, heap, cnt, bytes;
+  size_t tA, nc, xn, en, heap, cnt, bytes;
   __host__ __device__ FilterLayout(int KS, int CB, int Q, int k) {
     const size_t tile = (size_t)KS * CB * 64 * 8;  // bf16 elements
     const size_t C = 16 * (size_t)CB;
     tA = 0;
-    nc = al16r(tA + 2 * tile * 2);
-    xn = al16r(nc + 2 * C * 4);
-    en = al16r(xn + 2 * C * 4);
-    pos = al16r(en + 2 * C * 4);
-    heap = al16r(pos + 2 * C * 4);
+    nc = al16r(tA + tile * 2);
+    xn = al16r(nc + C * 4);
+    en = al16r(xn + C * 4);
+    heap = al16r(en + C * 4);
     cnt = al16r(heap + (size_t)k * Q * 4);
     bytes = al16r(cnt + (size_t)Q * 4);
   }
@@ -173,13 +172,28 @@ __device__ void sift_vals(float *h, int QS, int q, int k, float x) {
   h[s * QS + q] = x;
 }
 
+// Candidate tiles go global -> registers -> LDS (one buffer, two barriers per
+// tile): thread t stages the 16-byte chunks [ch0, ch0 + CPT) of candidate slot
+// t / TPC; the amap entry of the next tile is read one tile ahead.  Lanes hold
+// their query fragments in registers for the whole launch.
+//
+// Hot path per pair: tt = fma(-2, dot~, nc) and one compare against the
+// query's threshold thr = U - nq + delta_max (delta_max bounds delta over all
+// candidates).  Survivors get their exact lower/upper bounds and wait in a
+// per-lane queue of QR entries; the queue is applied (list append, upper-bound
+// heap update) only when some lane's queue would overflow.  A stale U only
+// admits more candidates; every queued candidate is tested against the live U
+// when applied.
 template <int KS, int QB, int CB, int W>
 __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
   constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, NT = 64 * W;
   constexpr int DP = 32 * KS;
-  constexpr int TILE = KS * CB * 64 * 8;      // bf16 elements per tile
-  constexpr int CHUNKS = C * (DP / 8);        // 16-byte chunks per tile
-  constexpr int LPT = (CHUNKS + NT - 1) / NT;
+  constexpr int TILE = KS * CB * 64 * 8;  // bf16 elements per tile
+  constexpr int CH = DP / 8;              // 16-byte chunks per candidate row
+  static_assert(NT % C == 0, "threads per candidate");
+  constexpr int TPC = NT / C;
+  constexpr int CPT = (CH + TPC - 1) / TPC;
+  constexpr int QR = 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k;
   const FilterLayout L(KS, CB, Q, k);
@@ -187,7 +201,6 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
   float *tnc = (float *)(smem + L.nc);
   float *txn = (float *)(smem + L.xn);
   float *ten = (float *)(smem + L.en);
-  int *tpos = (int *)(smem + L.pos);
   float *hv = (float *)(smem + L.heap);
   int *hc = (int *)(smem + L.cnt);
 
@@ -212,7 +225,6 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
       const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
       for (int t = 0; t < k; ++t) hv[t * Q + tid] = __double2float_ru(a.D[row + t]);
       for (int s = k / 2 - 1; s >= 0; --s) {
-        // bottom-up heapify of a values-only max-heap
         float x = hv[s * Q + tid];
         int p = s;
         while (true) {
@@ -230,7 +242,6 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     hc[tid] = 0;
   }
 
-  // Query fragments and per-query bound factors.
   const float u = kU32;
   const float g = (float)((a.d + 1) * 2.0 * 0x1p-24 / (1.0 - (a.d + 1) * 2.0 * 0x1p-24));
   const float cxx = 2.0f * g + 8.0f * u;
@@ -243,14 +254,15 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     ql[qb] = w * QW + qb * 16 + (lane & 15);
     const int j = qbase + ql[qb];
     qv[qb] = j < a.n;
-    const int row = qv[qb] ? a.bmap[j] : 0;
+    const int row = a.bmap[qv[qb] ? j : a.n - 1];
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
-      bq[qb][s] = qv[qb] ? *(const bf16x8 *)(xbB + (size_t)row * DP + s * 32 + 8 * grp)
-                         : bf16x8{};
-    const float xq = qv[qb] ? xnB[row] : 0.0f;
-    const float eq = qv[qb] ? enB[row] : 0.0f;
-    nq[qb] = qv[qb] ? (float)n64B[row] : 0.0f;
+    for (int s = 0; s < KS; ++s) {
+      const bf16x8 v = *(const bf16x8 *)(xbB + (size_t)row * DP + s * 32 + 8 * grp);
+      bq[qb][s] = qv[qb] ? v : bf16x8{};
+    }
+    const float xq = xnB[row];
+    const float eq = enB[row];
+    nq[qb] = (float)n64B[row];
     al[qb] = 2.0f * eq + cxx * xq;
     be[qb] = 2.0f * xq + 2.0f * eq;
     cq[qb] = 4.0f * u * nq[qb];
@@ -258,80 +270,119 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
   }
 
   const int ntiles = (a.m + C - 1) / C;
-  uint4 stage[LPT];
-  auto load_tile = [&](int t) {
-    const int c0 = t * C;
-    const int cnt = min(C, a.m - c0);
+  const int cs = tid / TPC;
+  const int ch0 = (tid % TPC) * CPT;
+  uint4 stage[CPT];
+  float st_n = 0.f, st_x = 0.f, st_e = 0.f;
+  auto cand_pos = [&](int t) { return min(t * C + cs, a.m - 1); };
+  auto load_x = [&](int id) {
+    const uint4 *row = (const uint4 *)(xbA + (size_t)id * DP);
 #pragma unroll
-    for (int uu = 0; uu < LPT; ++uu) {
-      const int e = tid + uu * NT;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (e < CHUNKS) {
-        const int c = e / (DP / 8), ch = e % (DP / 8);
-        if (c < cnt) v = *(const uint4 *)(xbA + (size_t)a.amap[c0 + c] * DP + ch * 8);
-      }
-      stage[uu] = v;
+    for (int e = 0; e < CPT; ++e) {
+      const int ch = ch0 + e;
+      stage[e] = row[(CH % TPC == 0 || ch < CH) ? ch : CH - 1];
     }
+    st_n = (float)n64A[id];
+    st_x = xnA[id];
+    st_e = enA[id];
   };
-  auto store_tile = [&](int t, int buf) {
-    const int c0 = t * C;
-    const int cnt = min(C, a.m - c0);
-    __bf16 *A = tA + buf * TILE;
+  auto store_x = [&](int t) {
+    const bool ok = t * C + cs < a.m;
+    const int cb = cs >> 4, i = cs & 15;
 #pragma unroll
-    for (int uu = 0; uu < LPT; ++uu) {
-      const int e = tid + uu * NT;
-      if (e < CHUNKS) {
-        const int c = e / (DP / 8), ch = e % (DP / 8);
-        const int s = ch >> 2, gg = ch & 3, cb = c >> 4, i = c & 15;
-        *(uint4 *)(A + ((size_t)(s * CB + cb) * 64 + i + 16 * gg) * 8) = stage[uu];
+    for (int e = 0; e < CPT; ++e) {
+      const int ch = ch0 + e;
+      if (CH % TPC == 0 || ch < CH) {
+        const int s = ch >> 2, gg = ch & 3;
+        *(uint4 *)(tA + ((size_t)(s * CB + cb) * 64 + i + 16 * gg) * 8) =
+            ok ? stage[e] : make_uint4(0, 0, 0, 0);
       }
     }
-    if (tid < C) {
-      const bool ok = tid < cnt;
-      const int id = ok ? a.amap[c0 + tid] : 0;
-      tnc[buf * C + tid] = ok ? (float)n64A[id] : __builtin_inff();
-      txn[buf * C + tid] = ok ? xnA[id] : 0.0f;
-      ten[buf * C + tid] = ok ? enA[id] : 0.0f;
-      tpos[buf * C + tid] = ok ? c0 + tid : -1;
+    if (tid % TPC == 0) {
+      tnc[cs] = ok ? st_n : __builtin_inff();
+      txn[cs] = ok ? st_x : 0.0f;
+      ten[cs] = ok ? st_e : 0.0f;
     }
   };
 
-  __syncthreads();
-  float U[QB], thr[QB];
-#pragma unroll
-  for (int qb = 0; qb < QB; ++qb) {
-    U[qb] = qv[qb] ? hv[ql[qb]] : -__builtin_inff();
-    thr[qb] = U[qb] - nq[qb] + dmx[qb];
-  }
-
+  int id_n = 0;
   if (ntiles > 0) {
-    load_tile(0);
-    store_tile(0, 0);
+    const int id0 = a.amap[cand_pos(0)];
+    load_x(id0);
+    id_n = a.amap[cand_pos(ntiles > 1 ? 1 : 0)];
+    __syncthreads();  // heaps loaded
+    store_x(0);
   }
   __syncthreads();
+  float thr[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb)
+    thr[qb] = qv[qb] ? hv[ql[qb]] - nq[qb] + dmx[qb] : -__builtin_inff();
+
+  // Per-lane queue: lower, upper, candidate position, query block.
+  float pq_lo[QR], pq_up[QR];
+  int pq_ps[QR], pq_q[QR];
+  int pq_n = 0;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+    pq_lo[r] = pq_up[r] = 0.f;
+    pq_ps[r] = pq_q[r] = 0;
+  }
+  // Apply one candidate to query slot `slot`: append if it can still reach
+  // the k-th distance, tighten U with its upper bound.
+  auto apply = [&](int slot, float lower, float upper, int ps) {
+    const float Uq = hv[slot];
+    if (lower <= Uq) {
+      const int at = hc[slot]++;
+      if (at < a.cap)
+        a.lst[(size_t)(qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+    }
+    if (upper < Uq) sift_vals(hv, Q, slot, k, upper);
+  };
+  auto flush = [&]() {
+#pragma unroll 1
+    for (int gsel = 0; gsel < 4; ++gsel) {
+      if (grp == gsel) {
+        for (int r = 0; r < QR; ++r) {
+          if (r < pq_n) {
+            int slot = ql[0];
+#pragma unroll
+            for (int qb = 1; qb < QB; ++qb)
+              if (pq_q[r] == qb) slot = ql[qb];
+            apply(slot, pq_lo[r], pq_up[r], pq_ps[r]);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+    pq_n = 0;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+      if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+  };
 
   for (int t = 0; t < ntiles; ++t) {
-    const int buf = t & 1;
-    if (t + 1 < ntiles) load_tile(t + 1);
+    const bool more = t + 1 < ntiles;
+    const int id_nn = (t + 2 < ntiles) ? a.amap[cand_pos(t + 2)] : id_n;
+    if (more) load_x(id_n);
     f32x4 acc[QB][CB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const __bf16 *A = tA + buf * TILE;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
-        const bf16x8 av = *(const bf16x8 *)(A + ((size_t)(s * CB + cb) * 64 + lane) * 8);
+        const bf16x8 av = *(const bf16x8 *)(tA + ((size_t)(s * CB + cb) * 64 + lane) * 8);
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
-    const float *NC = tnc + buf * C;
     unsigned pass[QB];
-    bool anyp = false;
+    int npass = 0;
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
       pass[qb] = 0u;
@@ -340,66 +391,82 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int c = cb * 16 + out_row(lane, v);
-          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], NC[c]);
+          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
           acc[qb][cb][v] = tt;
-          if (tt <= thr[qb]) {
-            pass[qb] |= 1u << (cb * 4 + v);
-            anyp = true;
-          }
+          if (tt <= thr[qb]) pass[qb] |= 1u << (cb * 4 + v);
         }
+      npass += __builtin_popcount(pass[qb]);
     }
-    if (__any(anyp)) {
-      const float *XN = txn + buf * C;
-      const float *EN = ten + buf * C;
-      const int *PS = tpos + buf * C;
+    if (__any(npass > 0)) {
+      if (__any(pq_n + npass > QR)) flush();
+      // After a flush the queues are empty; if some lane still cannot queue
+      // all of its survivors, every lane applies this tile's survivors now.
+      const bool direct = pq_n + npass > QR;
+      if (__any(direct)) {
+#pragma unroll 1
+        for (int gsel = 0; gsel < 4; ++gsel) {
+          if (grp == gsel) {
 #pragma unroll
-      for (int gsel = 0; gsel < 4; ++gsel) {
-        if (grp == gsel) {
+            for (int qb = 0; qb < QB; ++qb) {
+              unsigned bits = pass[qb];
+              while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1;
+                float tt = acc[qb][0][0];
 #pragma unroll
-          for (int qb = 0; qb < QB; ++qb) {
-            if (!pass[qb]) continue;
-            const int slot = ql[qb];
-            const int j = qbase + slot;
-            float Uq = hv[slot];
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                if (!(pass[qb] & (1u << (cb * 4 + v)))) continue;
-                const int c = cb * 16 + out_row(lane, v);
-                const int ps = PS[c];
-                if (ps < 0) continue;
-                const float dist = acc[qb][cb][v] + nq[qb];
-                const float delta =
-                    (XN[c] * al[qb] + EN[c] * be[qb] + 4.0f * u * NC[c] + cq[qb]) * 1.0009765625f +
-                    1e-30f;
-                const float lower = dist - delta;
-                const float upper = dist + delta;
-                if (lower <= Uq) {
-                  const int at = hc[slot]++;
-                  if (at < a.cap)
-                    a.lst[(size_t)j * a.cap + at] = make_int2(ps, __float_as_int(lower));
-                }
-                if (upper < Uq) {
-                  sift_vals(hv, Q, slot, k, upper);
-                  Uq = hv[slot];
-                }
+                for (int bb = 1; bb < CB * 4; ++bb)
+                  if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
+                const int c = (b >> 2) * 16 + out_row(lane, b & 3);
+                if (t * C + c >= a.m) continue;
+                const float dist = tt + nq[qb];
+                const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
+                                        1.0009765625f + 1e-30f;
+                apply(ql[qb], dist - delta, dist + delta, t * C + c);
               }
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+      } else {
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            float tt = acc[qb][0][0];
+#pragma unroll
+            for (int bb = 1; bb < CB * 4; ++bb)
+              if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
+            const int c = (b >> 2) * 16 + out_row(lane, b & 3);
+            if (t * C + c >= a.m) continue;
+            const float dist = tt + nq[qb];
+            const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
+                                    1.0009765625f + 1e-30f;
+#pragma unroll
+            for (int r = 0; r < QR; ++r)
+              if (pq_n == r) {
+                pq_lo[r] = dist - delta;
+                pq_up[r] = dist + delta;
+                pq_ps[r] = t * C + c;
+                pq_q[r] = qb;
+              }
+            ++pq_n;
           }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      }
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) {
-        if (!qv[qb]) continue;
-        U[qb] = hv[ql[qb]];
-        thr[qb] = U[qb] - nq[qb] + dmx[qb];
       }
     }
-    if (t + 1 < ntiles) store_tile(t + 1, buf ^ 1);
-    __syncthreads();
+    __syncthreads();  // tile consumed
+    if (more) store_x(t + 1);
+    __syncthreads();  // next tile visible
+    id_n = id_nn;
   }
+  if (__any(pq_n > 0)) flush();
+  __syncthreads();
 
   if (tid < Q) {
     const int j = qbase + tid;
