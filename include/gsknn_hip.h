@@ -23,10 +23,15 @@
 extern "C" {
 #endif
 
-/* Contraction selection for the FP64 path. */
-#define GSKNN_MODE_AUTO 0  /* MFMA when d >= 32, exact VALU below */
-#define GSKNN_MODE_EXACT 1 /* VALU, reference oracle arithmetic, bit-exact */
-#define GSKNN_MODE_MFMA 2  /* FP64 MFMA contraction */
+/* Contraction selection.
+ * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
+ * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA.
+ * FP32: AUTO/MFMA = FP32 MFMA candidate pass + re-rank in the oracle's FP32
+ * arithmetic when d >= 8 and n >= 4096, else the exact VALU kernel; EXACT =
+ * always the VALU kernel.  Every FP32 mode returns the oracle's distance bits. */
+#define GSKNN_MODE_AUTO 0
+#define GSKNN_MODE_EXACT 1
+#define GSKNN_MODE_MFMA 2
 
 /* Error codes. */
 #define GSKNN_OK 0
@@ -51,12 +56,12 @@ int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA,
                    int ldk, const int *hrow, int mode, void *work,
                    size_t work_bytes, void *stream);
 
-size_t gsknn_sknn_work_bytes(int m, int n, int d, int k);
+size_t gsknn_sknn_work_bytes(int m, int n, int d, int k, int mode);
 int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
                    const float *XA2, const int *amap, const float *XB,
                    const float *XB2, const int *bmap, float *D, int *I,
-                   int ldk, const int *hrow, void *work, size_t work_bytes,
-                   void *stream);
+                   int ldk, const int *hrow, int mode, void *work,
+                   size_t work_bytes, void *stream);
 
 /* Exact kNN from a BF16 MFMA candidate pass and an FP64 re-rank.
  *

@@ -19,7 +19,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgsknn.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_sweep.hip", "gsknn_api.cpp",
+SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip", "gsknn_api.cpp",
            "gsknn_serve.cpp"]
 
 HIPFLAGS = [

@@ -105,8 +105,8 @@ extern "C" size_t gsknn_dknn_work_bytes(int m, int n, int d, int k, int mode) {
   return knn_work_bytes(m, n, d, k, 8, mode);
 }
 
-extern "C" size_t gsknn_sknn_work_bytes(int m, int n, int d, int k) {
-  return knn_work_bytes(m, n, d, k, 4, MODE_EXACT);
+extern "C" size_t gsknn_sknn_work_bytes(int m, int n, int d, int k, int mode) {
+  return knn_work_bytes(m, n, d, k, 4, mode);
 }
 
 extern "C" int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA, const double *XA2,
@@ -121,9 +121,10 @@ extern "C" int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA, cons
 extern "C" int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA, const float *XA2,
                               const int *amap, const float *XB, const float *XB2,
                               const int *bmap, float *D, int *I, int ldk, const int *hrow,
-                              void *work, size_t work_bytes, void *stream) {
+                              int mode, void *work, size_t work_bytes, void *stream) {
+  if (mode < 0 || mode > 2) return set_err(GSKNN_EINVAL, "gsknn_sknn_dev: mode %d", mode);
   return knn_dev("gsknn_sknn_dev", m, n, d, k, XA, XA2, amap, XB, XB2, bmap, D, I, ldk, hrow,
-                 MODE_EXACT, work, work_bytes, stream);
+                 mode, work, work_bytes, stream);
 }
 
 extern "C" int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream) {
@@ -488,7 +489,7 @@ extern "C" void dgsknn_var1(int m, int n, int k, int r, double *XA, double *XA2,
 extern "C" void sgsknn_var1(int m, int n, int k, int r, float *XA, float *XA2, int *amap,
                             float *XB, float *XB2, int *bmap, float *D, int *I) {
   g_err[0] = 0;
-  (void)knn_host("sgsknn_var1", n, m, k, r, XB, XB2, bmap, XA, XA2, amap, D, I, r, MODE_EXACT);
+  (void)knn_host("sgsknn_var1", n, m, k, r, XB, XB2, bmap, XA, XA2, amap, D, I, r, env_mode());
 }
 
 extern "C" void sgsknn(int m, int n, int k, int r, float *XA, float *XA2, int *alpha, float *XB,

@@ -53,6 +53,13 @@ hipError_t launch_hdr_to_rows(int n, int k, int ldk, const double *Dh, const int
 hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int *It,
                               double *Dh, int *Ih, hipStream_t s);
 
+// FP32 MFMA candidate pass + re-rank in the oracle's FP32 arithmetic
+// (gsknn_f32.hip).
+size_t f32_work_bytes(int m, int n, int d, int k);
+bool f32_mfma_supported(int d, int k);
+hipError_t launch_knn_f32_mfma(const KnnArgs<float> &a, hipStream_t s, void *work,
+                               size_t work_bytes, int *overflow_out);
+
 // BF16 candidate pass + FP64 re-rank (gsknn_rerank.hip).
 size_t bf16_prep_bytes(int N, int d);
 hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_t s);

@@ -1,0 +1,628 @@
+// FP32 exact kNN (sgsknn) from an FP32 MFMA candidate pass and a re-rank in
+// the reference oracle's own arithmetic.
+//
+// Reference: sgsknn_ref (frame/gsknn_ref.c:63-109, frame/sgsknn_ref_impl.h:
+// 41-103): distance = ((-2 * sum_p a_p b_p) + XA2[amap[i]]) + XB2[bmap[j]] with
+// the dot product summed in p order in FP32, every product and sum rounded
+// separately; per heap row the k smallest are kept (frame/gsknn_heap.c:104-130).
+//
+// Pass 1 (knn_f32_filter_kernel) computes dist~ with v_mfma_f32_16x16x4_f32
+// (FP32 products and sums in the matrix core's own order) and brackets the
+// oracle's distance d_or by
+//   |dist~ - d_or| <= delta = c_d |a| |b| + 4u (|nc| + |nq|),
+//   c_d = (2 g_m + 2 g_o + 8.1 u),  g_m = (2d+2) 2u / (1 - (2d+2) 2u),
+//   g_o = d u / (1 - d u),  u = 2^-24,
+// where |a|, |b| are the true Euclidean norms of the points (computed in FP64,
+// rounded up), g_m covers the MFMA sum (each product or addition rounding to
+// within 2u, any order), g_o the oracle's sequential sum, and the 4u terms the
+// two FP32 epilogue additions on each side.  As in the BF16 form, U = k-th
+// smallest upper bound per query, and every candidate whose lower bound is <= U
+// when seen is listed.
+// Pass 2 (knn_f32_rerank_kernel) recomputes the listed candidates with the
+// oracle's arithmetic (one lane per candidate, p in order, no fused
+// multiply-add) and merges them into the FP32 heap row under the canonical
+// (distance, index) order: the distances are bit-identical to sgsknn_ref's.
+#include <limits.h>
+
+#include "gsknn_common.h"
+
+namespace gsknn {
+
+namespace {
+
+__host__ __device__ constexpr size_t a16(size_t x) { return (x + 15) & ~size_t(15); }
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kU = 5.9604644775390625e-08f;  // 2^-24
+
+__device__ __forceinline__ bool cl_f(float da, int ia, float db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+__device__ void sift_v(float *h, int QS, int q, int k, float x) {
+  int s = 0;
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    float cv = h[c * QS + q];
+    if (c + 1 < k) {
+      const float rv = h[(c + 1) * QS + q];
+      if (cv < rv) {
+        ++c;
+        cv = rv;
+      }
+    }
+    if (!(x < cv)) break;
+    h[s * QS + q] = cv;
+    s = c;
+  }
+  h[s * QS + q] = x;
+}
+
+__device__ void sift_fi(float *hd, int *hi, int s, int k, float xd, int xi) {
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    float cd = hd[c];
+    int ci = hi[c];
+    if (c + 1 < k && cl_f(cd, ci, hd[c + 1], hi[c + 1])) {
+      ++c;
+      cd = hd[c];
+      ci = hi[c];
+    }
+    if (!cl_f(xd, xi, cd, ci)) break;
+    hd[s] = cd;
+    hi[s] = ci;
+    s = c;
+  }
+  hd[s] = xd;
+  hi[s] = xi;
+}
+
+}  // namespace
+
+struct F32Args {
+  int m, n, d, k, ldk, cap;
+  const float *XA, *XA2;
+  const int *amap;
+  const float *XB, *XB2;
+  const int *bmap;
+  float *D;
+  int *I;
+  const int *hrow;
+  float *xnA;   // [m] |XA[amap[i]]| rounded up
+  float *xnB;   // [n] |XB[bmap[j]]| rounded up
+  float *mx;    // {max xnA, max |XA2[amap[i]]|}
+  int2 *lst;    // [n][cap]
+  int *cnt;     // [n]
+  float *U;     // [n]
+  int *nover;
+};
+
+// Norms of the gathered points (FP64 sum, rounded up) and candidate maxima.
+__global__ void f32_prep_kernel(F32Args a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.m) {
+    const float *x = a.XA + (size_t)a.amap[i] * a.d;
+    double s = 0.0;
+    for (int p = 0; p < a.d; ++p) s += (double)x[p] * (double)x[p];
+    const float xn = __double2float_ru(sqrt(s) * (1.0 + 0x1p-40));
+    a.xnA[i] = xn;
+    atomicMax((int *)&a.mx[0], __float_as_int(xn));
+    atomicMax((int *)&a.mx[1], __float_as_int(fabsf(a.XA2[a.amap[i]])));
+  } else if (i < a.m + a.n) {
+    const int j = i - a.m;
+    const float *x = a.XB + (size_t)a.bmap[j] * a.d;
+    double s = 0.0;
+    for (int p = 0; p < a.d; ++p) s += (double)x[p] * (double)x[p];
+    a.xnB[j] = __double2float_ru(sqrt(s) * (1.0 + 0x1p-40));
+  }
+}
+
+__global__ void f32_zero_kernel(float *mx, int *nover) {
+  if (threadIdx.x < 2) mx[threadIdx.x] = 0.0f;
+  if (threadIdx.x == 0) *nover = 0;
+}
+
+struct F32Layout {
+  size_t tA, nc, xn, heap, cnt, bytes;
+  __host__ __device__ F32Layout(int KS, int CB, int Q, int k) {
+    const size_t tile = (size_t)KS * CB * 64;  // floats
+    const size_t C = 16 * (size_t)CB;
+    tA = 0;
+    nc = a16(tile * 4);
+    xn = a16(nc + C * 4);
+    heap = a16(xn + C * 4);
+    cnt = a16(heap + (size_t)k * Q * 4);
+    bytes = a16(cnt + (size_t)Q * 4);
+  }
+};
+
+// Same structure as the BF16 filter: candidate tiles global -> registers ->
+// one LDS buffer in MFMA fragment order (lane l of k-step s, block cb holds
+// candidate cb*16 + l%16, dim 4s + l/16), query fragments in registers, a
+// one-compare hot filter and a per-lane deferred queue for the survivors.
+template <int KS, int QB, int CB, int W>
+__global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, NT = 64 * W;
+  constexpr int DP = 4 * KS;
+  constexpr int CH = KS;  // 16-byte chunks (4 dims) per candidate row
+  static_assert(NT % C == 0, "threads per candidate");
+  constexpr int TPC = NT / C;
+  constexpr int CPT = (CH + TPC - 1) / TPC;
+  constexpr int QR = 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k, d = a.d;
+  const F32Layout L(KS, CB, Q, k);
+  float *tA = (float *)(smem + L.tA);
+  float *tnc = (float *)(smem + L.nc);
+  float *txn = (float *)(smem + L.xn);
+  float *hv = (float *)(smem + L.heap);
+  int *hc = (int *)(smem + L.cnt);
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4;
+  const int qbase = blockIdx.x * Q;
+
+  if (tid < Q) {
+    const int j = qbase + tid;
+    if (j < a.n) {
+      const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+      for (int t = 0; t < k; ++t) hv[t * Q + tid] = a.D[row + t];
+      for (int s = k / 2 - 1; s >= 0; --s) {
+        float x = hv[s * Q + tid];
+        int p = s;
+        while (true) {
+          int c = 2 * p + 1;
+          if (c >= k) break;
+          float cv = hv[c * Q + tid];
+          if (c + 1 < k && cv < hv[(c + 1) * Q + tid]) cv = hv[(++c) * Q + tid];
+          if (!(x < cv)) break;
+          hv[p * Q + tid] = cv;
+          p = c;
+        }
+        hv[p * Q + tid] = x;
+      }
+    }
+    hc[tid] = 0;
+  }
+
+  const float u = kU;
+  const double gm = (2.0 * d + 2) * 2.0 * 0x1p-24 / (1.0 - (2.0 * d + 2) * 2.0 * 0x1p-24);
+  const double go = d * 0x1p-24 / (1.0 - d * 0x1p-24);
+  const float cd = (float)((2.0 * gm + 2.0 * go + 8.1 * 0x1p-24) * 1.01);
+  float bq[QB][KS];
+  float nq[QB], al[QB], cq[QB], dmx[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + (lane & 15);
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const int jj = qv[qb] ? j : a.n - 1;
+    const float *xq = a.XB + (size_t)a.bmap[jj] * d;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int p = 4 * s + grp;
+      const float v = xq[p < d ? p : d - 1];
+      bq[qb][s] = (qv[qb] && p < d) ? v : 0.0f;
+    }
+    nq[qb] = a.XB2[a.bmap[jj]];
+    al[qb] = cd * a.xnB[jj];
+    cq[qb] = 4.0f * u * fabsf(nq[qb]);
+    dmx[qb] = (al[qb] * a.mx[0] + 4.0f * u * a.mx[1] + cq[qb]) * 1.002f + 1e-30f;
+  }
+
+  const int ntiles = (a.m + C - 1) / C;
+  const int cs = tid / TPC;
+  const int ch0 = (tid % TPC) * CPT;
+  float4 stage[CPT];
+  float st_n = 0.f, st_x = 0.f;
+  auto cand_pos = [&](int t) { return min(t * C + cs, a.m - 1); };
+  auto load_x = [&](int pos, int id) {
+    const float *row = a.XA + (size_t)id * d;
+#pragma unroll
+    for (int e = 0; e < CPT; ++e) {
+      const int ch = ch0 + e;
+      float4 v;
+      const int p = 4 * ch;
+      v.x = row[min(p, d - 1)];
+      v.y = row[min(p + 1, d - 1)];
+      v.z = row[min(p + 2, d - 1)];
+      v.w = row[min(p + 3, d - 1)];
+      stage[e] = v;
+    }
+    st_n = a.XA2[id];
+    st_x = a.xnA[pos];
+  };
+  auto store_x = [&](int t) {
+    const bool ok = t * C + cs < a.m;
+    const int cb = cs >> 4, i = cs & 15;
+#pragma unroll
+    for (int e = 0; e < CPT; ++e) {
+      const int ch = ch0 + e;
+      if (CH % TPC == 0 || ch < CH) {
+        float *A = tA + (size_t)(ch * CB + cb) * 64 + i;
+        const int p = 4 * ch;
+        A[0] = (ok && p < d) ? stage[e].x : 0.0f;
+        A[16] = (ok && p + 1 < d) ? stage[e].y : 0.0f;
+        A[32] = (ok && p + 2 < d) ? stage[e].z : 0.0f;
+        A[48] = (ok && p + 3 < d) ? stage[e].w : 0.0f;
+      }
+    }
+    if (tid % TPC == 0) {
+      tnc[cs] = ok ? st_n : __builtin_inff();
+      txn[cs] = ok ? st_x : 0.0f;
+    }
+  };
+
+  int pos_n = 0, id_n = 0;
+  if (ntiles > 0) {
+    load_x(cand_pos(0), a.amap[cand_pos(0)]);
+    pos_n = cand_pos(ntiles > 1 ? 1 : 0);
+    id_n = a.amap[pos_n];
+    __syncthreads();
+    store_x(0);
+  }
+  __syncthreads();
+  float thr[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb)
+    thr[qb] = qv[qb] ? hv[ql[qb]] - nq[qb] + dmx[qb] : -__builtin_inff();
+
+  float pq_lo[QR], pq_up[QR];
+  int pq_ps[QR], pq_q[QR];
+  int pq_n = 0;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+    pq_lo[r] = pq_up[r] = 0.f;
+    pq_ps[r] = pq_q[r] = 0;
+  }
+  auto apply = [&](int slot, float lower, float upper, int ps) {
+    const float Uq = hv[slot];
+    if (lower <= Uq) {
+      const int at = hc[slot]++;
+      if (at < a.cap) a.lst[(size_t)(qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+    }
+    if (upper < Uq) sift_v(hv, Q, slot, k, upper);
+  };
+  auto refresh = [&]() {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+      if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+  };
+  auto flush = [&]() {
+#pragma unroll 1
+    for (int gsel = 0; gsel < 4; ++gsel) {
+      if (grp == gsel) {
+        for (int r = 0; r < QR; ++r) {
+          if (r < pq_n) {
+            int slot = ql[0];
+#pragma unroll
+            for (int qb = 1; qb < QB; ++qb)
+              if (pq_q[r] == qb) slot = ql[qb];
+            apply(slot, pq_lo[r], pq_up[r], pq_ps[r]);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+    pq_n = 0;
+    refresh();
+  };
+
+  for (int t = 0; t < ntiles; ++t) {
+    const bool more = t + 1 < ntiles;
+    const int pos_nn = cand_pos(t + 2 < ntiles ? t + 2 : t + 1);
+    const int id_nn = a.amap[pos_nn];
+    if (more) load_x(pos_n, id_n);
+    f32x4 acc[QB][CB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const float av = tA[(size_t)(s * CB + cb) * 64 + lane];
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+      }
+    }
+    unsigned pass[QB];
+    int npass = 0;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      pass[qb] = 0u;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = cb * 16 + 4 * grp + v;
+          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
+          acc[qb][cb][v] = tt;
+          if (tt <= thr[qb]) pass[qb] |= 1u << (cb * 4 + v);
+        }
+      npass += __builtin_popcount(pass[qb]);
+    }
+    if (__any(npass > 0)) {
+      if (__any(pq_n + npass > QR)) flush();
+      const bool direct = __any(pq_n + npass > QR);
+#pragma unroll 1
+      for (int gsel = 0; gsel < (direct ? 4 : 1); ++gsel) {
+        if (!direct || grp == gsel) {
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            unsigned bits = pass[qb];
+            while (bits) {
+              const int b = __builtin_ctz(bits);
+              bits &= bits - 1;
+              float tt = acc[qb][0][0];
+#pragma unroll
+              for (int bb = 1; bb < CB * 4; ++bb)
+                if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
+              const int c = (b >> 2) * 16 + 4 * grp + (b & 3);
+              if (t * C + c >= a.m) continue;
+              const float dist = tt + nq[qb];
+              const float delta = txn[c] * al[qb] + 4.0f * u * fabsf(tnc[c]) + cq[qb] + 1e-30f;
+              if (direct) {
+                apply(ql[qb], dist - delta, dist + delta, t * C + c);
+              } else {
+#pragma unroll
+                for (int r = 0; r < QR; ++r)
+                  if (pq_n == r) {
+                    pq_lo[r] = dist - delta;
+                    pq_up[r] = dist + delta;
+                    pq_ps[r] = t * C + c;
+                    pq_q[r] = qb;
+                  }
+                ++pq_n;
+              }
+            }
+          }
+        }
+        if (direct) {
+          __builtin_amdgcn_wave_barrier();
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+      }
+      if (direct) refresh();
+    }
+    __syncthreads();
+    if (more) store_x(t + 1);
+    __syncthreads();
+    pos_n = pos_nn;
+    id_n = id_nn;
+  }
+  if (__any(pq_n > 0)) flush();
+  __syncthreads();
+
+  if (tid < Q) {
+    const int j = qbase + tid;
+    if (j < a.n) {
+      a.U[j] = hv[tid];
+      a.cnt[j] = hc[tid];
+      if (hc[tid] > a.cap) atomicAdd(a.nover, 1);
+    }
+  }
+}
+
+// The oracle's FP32 distance for candidate position pos and query j.
+__device__ __forceinline__ float oracle_dist_f32(const F32Args &a, int pos, int j) {
+  const float *xa = a.XA + (size_t)a.amap[pos] * a.d;
+  const float *xb = a.XB + (size_t)a.bmap[j] * a.d;
+  float s = 0.0f;
+  for (int p = 0; p < a.d; ++p) s = __fadd_rn(s, __fmul_rn(xa[p], xb[p]));
+  s = __fmul_rn(s, -2.0f);
+  s = __fadd_rn(s, a.XA2[a.amap[pos]]);
+  return __fadd_rn(s, a.XB2[a.bmap[j]]);
+}
+
+// Pass 2: one wave per query.  Lanes take 64 listed candidates at a time,
+// compute their oracle distances independently, then lane 0 merges the ones
+// that beat the current root into the LDS heap in lane order.
+__global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  float *hd = (float *)smem + (size_t)w * k;
+  int *hi = (int *)(smem + a16(sizeof(float) * 4 * (size_t)k)) + (size_t)w * k;
+  if (j >= a.n) return;
+  const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  if (lane == 0) {
+    for (int t = 0; t < k; ++t) {
+      hd[t] = a.D[row + t];
+      hi[t] = a.I[row + t];
+    }
+    for (int s = k / 2 - 1; s >= 0; --s) sift_fi(hd, hi, s, k, hd[s], hi[s]);
+  }
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  const float Ulim = a.U[j];
+  const int cnt = a.cnt[j];
+  const bool over = cnt > a.cap;
+  const int nscan = over ? a.m : cnt;
+  for (int e0 = 0; e0 < nscan; e0 += 64) {
+    const int e = e0 + lane;
+    bool have = false;
+    int pos = 0;
+    if (e < nscan) {
+      if (over) {
+        pos = e;
+        have = true;
+      } else {
+        const int2 en = a.lst[(size_t)j * a.cap + e];
+        pos = en.x;
+        have = __int_as_float(en.y) <= Ulim;
+      }
+    }
+    float dist = 0.0f;
+    int id = 0;
+    if (have) {
+      dist = oracle_dist_f32(a, pos, j);
+      id = a.amap[pos];
+    }
+    // Merge in lane order; only lanes that may beat the root take a turn.
+    unsigned long long todo = __ballot(have && cl_f(dist, id, hd[0], hi[0]));
+    while (todo) {
+      const int src = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const float xd = __shfl(dist, src, 64);
+      const int xi = __shfl(id, src, 64);
+      if (lane == 0 && cl_f(xd, xi, hd[0], hi[0])) sift_fi(hd, hi, 0, k, xd, xi);
+      __builtin_amdgcn_wave_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+  }
+  if (lane == 0) {
+    for (int e2 = k - 1; e2 > 0; --e2) {
+      const float xd = hd[e2];
+      const int xi = hi[e2];
+      hd[e2] = hd[0];
+      hi[e2] = hi[0];
+      sift_fi(hd, hi, 0, e2, xd, xi);
+    }
+    for (int t = 0; t < k; ++t) {
+      a.D[row + t] = hd[k - 1 - t];
+      a.I[row + t] = hi[k - 1 - t];
+    }
+  }
+}
+
+namespace {
+
+constexpr size_t kLdsF = 160 * 1024;
+
+struct F32Plan {
+  int ks = 0, qb = 0, cb = 0, w = 0, q = 0, cap = 0;
+  size_t lds = 0;
+};
+
+F32Plan plan_f32(int d, int k) {
+  F32Plan p;
+  const int need = (d + 3) / 4;
+  for (int ks : {4, 8, 16, 32, 64}) {
+    if (need <= ks) {
+      p.ks = ks;
+      break;
+    }
+  }
+  if (p.ks == 0) return p;
+  p.cb = p.ks >= 32 ? 1 : 2;
+  const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
+  for (auto &o : opts) {
+    const int Q = o[1] * 16 * o[0];
+    const size_t b = F32Layout(p.ks, p.cb, Q, k).bytes;
+    if (b <= kLdsF) {
+      p.qb = o[0];
+      p.w = o[1];
+      p.q = Q;
+      p.lds = b;
+      break;
+    }
+  }
+  const int cap = 16 * k;
+  p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
+  return p;
+}
+
+struct F32Work {
+  size_t xnA, xnB, mx, lst, cnt, U, nover, bytes;
+  F32Work(int m, int n, int cap) {
+    xnA = 0;
+    xnB = a16(xnA + (size_t)m * 4);
+    mx = a16(xnB + (size_t)n * 4);
+    lst = a16(mx + 16);
+    cnt = a16(lst + (size_t)n * cap * 8);
+    U = a16(cnt + (size_t)n * 4);
+    nover = a16(U + (size_t)n * 4);
+    bytes = a16(nover + 16);
+  }
+};
+
+template <int KS>
+hipError_t launch_f32_ks(const F32Plan &p, const F32Args &a, hipStream_t s) {
+  const dim3 grid((a.n + p.q - 1) / p.q);
+  constexpr int CBk = KS >= 32 ? 1 : 2;
+#define GSKNN_F32_CASE(QB_, W_)                                                        \
+  if (p.qb == QB_ && p.w == W_) {                                                      \
+    auto kern = knn_f32_filter_kernel<KS, QB_, CBk, W_>;                                \
+    hipError_t e = hipFuncSetAttribute((const void *)kern,                              \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                       (int)p.lds);                                     \
+    if (e != hipSuccess) return e;                                                      \
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);                         \
+    return hipGetLastError();                                                           \
+  }
+  GSKNN_F32_CASE(2, 4)
+  GSKNN_F32_CASE(1, 4)
+  GSKNN_F32_CASE(1, 2)
+  GSKNN_F32_CASE(1, 1)
+#undef GSKNN_F32_CASE
+  return hipErrorNotSupported;
+}
+
+}  // namespace
+
+size_t f32_work_bytes(int m, int n, int d, int k) {
+  const F32Plan p = plan_f32(d, k);
+  if (p.q == 0 || m <= 0 || n <= 0) return 0;
+  return F32Work(m, n, p.cap).bytes;
+}
+
+bool f32_mfma_supported(int d, int k) {
+  const F32Plan p = plan_f32(d, k);
+  const size_t rl = a16(sizeof(float) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;
+  return p.q > 0 && rl <= kLdsF;
+}
+
+hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *work,
+                               size_t work_bytes, int *overflow_out) {
+  const F32Plan p = plan_f32(ka.d, ka.k);
+  if (p.q == 0) return hipErrorNotSupported;
+  const size_t rl = a16(sizeof(float) * 4 * (size_t)ka.k) + sizeof(int) * 4 * (size_t)ka.k;
+  if (rl > kLdsF) return hipErrorNotSupported;
+  const F32Work W(ka.m, ka.n, p.cap);
+  if (work == nullptr || work_bytes < W.bytes) return hipErrorInvalidValue;
+  char *wb = (char *)work;
+  F32Args a{};
+  a.m = ka.m; a.n = ka.n; a.d = ka.d; a.k = ka.k; a.ldk = ka.ldk; a.cap = p.cap;
+  a.XA = ka.XA; a.XA2 = ka.XA2; a.amap = ka.amap;
+  a.XB = ka.XB; a.XB2 = ka.XB2; a.bmap = ka.bmap;
+  a.D = ka.D; a.I = ka.I; a.hrow = ka.hrow;
+  a.xnA = (float *)(wb + W.xnA);
+  a.xnB = (float *)(wb + W.xnB);
+  a.mx = (float *)(wb + W.mx);
+  a.lst = (int2 *)(wb + W.lst);
+  a.cnt = (int *)(wb + W.cnt);
+  a.U = (float *)(wb + W.U);
+  a.nover = (int *)(wb + W.nover);
+  hipLaunchKernelGGL(f32_zero_kernel, dim3(1), dim3(64), 0, s, a.mx, a.nover);
+  hipLaunchKernelGGL(f32_prep_kernel, dim3((a.m + a.n + 255) / 256), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (p.ks) {
+    case 4: e = launch_f32_ks<4>(p, a, s); break;
+    case 8: e = launch_f32_ks<8>(p, a, s); break;
+    case 16: e = launch_f32_ks<16>(p, a, s); break;
+    case 32: e = launch_f32_ks<32>(p, a, s); break;
+    case 64: e = launch_f32_ks<64>(p, a, s); break;
+    default: return hipErrorNotSupported;
+  }
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)knn_f32_rerank_kernel,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rl);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(knn_f32_rerank_kernel, dim3((a.n + 3) / 4), dim3(256), rl, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (overflow_out) e = hipMemcpyAsync(overflow_out, a.nover, sizeof(int), hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
+}  // namespace gsknn

@@ -69,11 +69,11 @@ def _declare(L):
     L.gsknn_last_error.restype = ctypes.c_char_p
     L.gsknn_dknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
     L.gsknn_dknn_dev.restype = i32
-    L.gsknn_sknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, vp, sz, vp]
+    L.gsknn_sknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
     L.gsknn_sknn_dev.restype = i32
     L.gsknn_dknn_work_bytes.argtypes = [i32] * 5
     L.gsknn_dknn_work_bytes.restype = sz
-    L.gsknn_sknn_work_bytes.argtypes = [i32] * 4
+    L.gsknn_sknn_work_bytes.argtypes = [i32] * 5
     L.gsknn_sknn_work_bytes.restype = sz
     L.gsknn_bf16_prep_bytes.argtypes = [i32, i32]
     L.gsknn_bf16_prep_bytes.restype = sz
@@ -219,13 +219,16 @@ def dknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work
         raise GsknnError(last_error())
 
 
-def sknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, work=None, stream=None):
+def sknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work=None,
+             stream=None):
+    """FP32 kNN on device tensors (sgsknn semantics); distances are the
+    oracle's FP32 bits in every mode."""
     k = D.shape[1]
     m, n, d = amap.numel(), bmap.numel(), XA.shape[1]
     wb = work.numel() * work.element_size() if work is not None else 0
     rc = lib().gsknn_sknn_dev(m, n, d, k, _dp(XA), _dp(XA2), _dp(amap), _dp(XB), _dp(XB2),
-                              _dp(bmap), _dp(D), _dp(I), D.stride(0), _dp(hrow), _dp(work), wb,
-                              _stream_ptr(stream))
+                              _dp(bmap), _dp(D), _dp(I), D.stride(0), _dp(hrow), mode, _dp(work),
+                              wb, _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
 
@@ -234,8 +237,8 @@ def work_bytes_d(m, n, d, k, mode=MODE_AUTO) -> int:
     return int(lib().gsknn_dknn_work_bytes(m, n, d, k, mode))
 
 
-def work_bytes_s(m, n, d, k) -> int:
-    return int(lib().gsknn_sknn_work_bytes(m, n, d, k))
+def work_bytes_s(m, n, d, k, mode=MODE_AUTO) -> int:
+    return int(lib().gsknn_sknn_work_bytes(m, n, d, k, mode))
 
 
 def sqnorm_dev(X, X2, stream=None):

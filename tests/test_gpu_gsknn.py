@@ -344,3 +344,38 @@ def test_bf16_carry_over_and_ties():
     for am in (amap1, amap2):
         oracle.gsknn(X64, X2, am, X64, X2, bmap, Dr, Ir, canonical=True)
     assert_rows(Dg, Ig, Dr, Ir)
+
+
+@pytest.mark.parametrize("d,k,lattice", [(24, 17, False), (100, 32, False), (256, 64, False),
+                                         (12, 10, True)])
+def test_sgsknn_fp32_mfma_rerank_bitwise(d, k, lattice):
+    # n >= 4096 takes the FP32 MFMA candidate pass + re-rank in the oracle's
+    # FP32 arithmetic; distances must still be the oracle's bits.
+    import torch
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(d * 7 + k))
+    N = 12000
+    if lattice:
+        X = np.floor(g.random((N, d)) * 5).astype(np.float32)
+    else:
+        X = g.standard_normal((N, d)).astype(np.float32)
+    X2 = oracle.sqnorm(X)
+    amap = g.integers(0, N, 6000).astype(np.int32)
+    bmap = g.permutation(N)[:4500].astype(np.int32)
+    D, I = ds.empty_heap(bmap.size, k, np.float32)
+    print(f"[gpu] sgsknn mfma d={d} k={k} lattice={lattice}", flush=True)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    tD, tI = torch.from_numpy(D).to(dev), torch.from_numpy(I).to(dev)
+    gsknn.sknn_dev(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2,
+                   torch.from_numpy(bmap).to(dev), tD, tI)
+    torch.cuda.synchronize()
+    Dg, Ig = tD.cpu().numpy(), tI.cpu().numpy()
+    # FP32 distances tie exactly now and then (ulp ~ 6e-5 at |D| ~ 500): the
+    # canonical oracle fixes the index choice; the reference's own rule must
+    # give the same distance multiset.
+    Dr, Ir = D.copy(), I.copy()
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(Dg, Ig, Dr, Ir)
+    Dc, Ic = D.copy(), I.copy()
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dc, Ic, canonical=False)
+    assert np.array_equal(_sorted(Dg, Ig)[0], _sorted(Dc, Ic)[0])
