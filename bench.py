@@ -206,12 +206,45 @@ def precision_form(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+def allnn_sweep(torch, dev, args):
+    """Config 4: N = 2^20 points, d = 32, FP32, 256-cluster mixture; 8
+    randomized-projection-tree iterations with leaves of 256-4096 points, k = 32,
+    global heaps carried across iterations.  One persistent launch per
+    iteration (graph-replayed after the first), autotuned launch shape."""
+    from rnn_amd import datasets as ds
+    from rnn_amd import sweep
+
+    N = (1 << 20) // args.scale
+    t0 = time.perf_counter()
+    Xn = ds.config4_points(N, 32, 256, seed=3)
+    log(f"c4 data N={N} d=32 ({time.perf_counter() - t0:.1f}s)")
+    X = torch.from_numpy(Xn).to(dev)
+    sw = sweep.RPTreeSweep(X, 32, seed=3, leaf_lo=256, leaf_hi=4096, log=log)
+    for _ in range(args.iters):
+        _, _, info = sw.step()
+        log(f"iteration {info['iteration']}: tree {info['tree_ms']:.1f} ms, sweep "
+            f"{info['sweep_ms']:.3f} ms, {info['leaves']} leaves {info['leaf_sizes']}")
+    t = sw.timings
+    pairs = sum(i["pairs"] for i in t)
+    sweep_ms = sum(i["sweep_ms"] for i in t)
+    out = {"workload": "c4", "N": N, "d": 32, "k": 32, "iterations": len(t),
+           "sweep_ms_per_iteration": [i["sweep_ms"] for i in t],
+           "tree_ms_per_iteration": [i["tree_ms"] for i in t],
+           "pairs_per_iteration": [i["pairs"] for i in t],
+           "pairs_per_s": pairs / (sweep_ms * 1e-3),
+           "graph_replayed_iterations": len(t) - 1,
+           "autotuned": {"qpb": sw.config[0], "ct": sw.config[1], "grid": sw.config[2],
+                         "tried": (sw.tune_info or {}).get("tried")}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "copy"])
+    ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
     ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma"])
@@ -233,6 +266,9 @@ def main():
         return
     if args.workload == "c3":
         precision_form(torch, dev, args)
+        return
+    if args.workload == "c4":
+        allnn_sweep(torch, dev, args)
         return
     t0 = time.perf_counter()
     p = ds.config2()

@@ -86,6 +86,26 @@ int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
                        void *work, size_t work_bytes, int *overflow,
                        void *stream);
 
+/* Leaf sweep of one randomized-projection-tree iteration, as one persistent
+ * launch.  perm holds the point ids grouped by leaf; items is an int array
+ * [nitems][4] = {leaf start in perm, leaf size, first query of the block, 0}
+ * with the first query a multiple of qpb, and *nitems (device memory) is the
+ * number of items, so one launch captured in a hipGraph can be replayed for
+ * every iteration.  For each item the queries perm[start + q0 .. + qpb) are
+ * matched against the candidates perm[start .. start + size) (gsknn with
+ * amap = bmap = the leaf), merging into heap row = point id (stride ldk).
+ * qpb in {16, 32, 64}; ct (candidates per LDS tile) a multiple of 4; grid =
+ * persistent workgroups.  Distances use the reference oracle's arithmetic. */
+size_t gsknn_sweep_lds_bytes(int elem_bytes, int d, int k, int qpb, int ct);
+int gsknn_sweep_leaves_sdev(int d, int k, const float *X, const float *X2,
+                            const int *perm, const int *items,
+                            const int *nitems, float *D, int *I, int ldk,
+                            int qpb, int ct, int grid, void *stream);
+int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
+                            const int *perm, const int *items,
+                            const int *nitems, double *D, int *I, int ldk,
+                            int qpb, int ct, int grid, void *stream);
+
 /* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
 int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);
 int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *stream);

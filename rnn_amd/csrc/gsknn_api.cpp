@@ -187,6 +187,41 @@ extern "C" int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const floa
   return e == hipSuccess ? GSKNN_OK : hip_err(e, fn);
 }
 
+extern "C" size_t gsknn_sweep_lds_bytes(int elem_bytes, int d, int k, int qpb, int ct) {
+  return leaf_sweep_lds_bytes(elem_bytes, d, k, qpb, ct);
+}
+
+template <typename T, typename F>
+static int sweep_dev(const char *fn, F launch, int d, int k, const T *X, const T *X2,
+                     const int *perm, const int *items, const int *nitems, T *D, int *I, int ldk,
+                     int qpb, int ct, int grid, void *stream) {
+  if (d <= 0 || k <= 0 || ldk < k) return set_err(GSKNN_EINVAL, "%s: d=%d k=%d ldk=%d", fn, d, k, ldk);
+  if (!X || !X2 || !perm || !items || !nitems || !D || !I)
+    return set_err(GSKNN_EINVAL, "%s: null array", fn);
+  hipError_t e = launch(d, k, X, X2, perm, items, nitems, D, I, ldk, qpb, ct, grid,
+                        (hipStream_t)stream);
+  if (e == hipErrorNotSupported)
+    return set_err(GSKNN_ELIMIT, "%s: qpb=%d ct=%d grid=%d does not fit (d=%d k=%d)", fn, qpb, ct,
+                   grid, d, k);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, fn);
+}
+
+extern "C" int gsknn_sweep_leaves_sdev(int d, int k, const float *X, const float *X2,
+                                       const int *perm, const int *items, const int *nitems,
+                                       float *D, int *I, int ldk, int qpb, int ct, int grid,
+                                       void *stream) {
+  return sweep_dev("gsknn_sweep_leaves_sdev", launch_leaf_sweep_f32, d, k, X, X2, perm, items,
+                   nitems, D, I, ldk, qpb, ct, grid, stream);
+}
+
+extern "C" int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
+                                       const int *perm, const int *items, const int *nitems,
+                                       double *D, int *I, int ldk, int qpb, int ct, int grid,
+                                       void *stream) {
+  return sweep_dev("gsknn_sweep_leaves_ddev", launch_leaf_sweep_f64, d, k, X, X2, perm, items,
+                   nitems, D, I, ldk, qpb, ct, grid, stream);
+}
+
 // ---------------------------------------------------------------------------
 // Host-pointer staging
 // ---------------------------------------------------------------------------

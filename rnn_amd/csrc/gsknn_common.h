@@ -60,6 +60,15 @@ bool f32_mfma_supported(int d, int k);
 hipError_t launch_knn_f32_mfma(const KnnArgs<float> &a, hipStream_t s, void *work,
                                size_t work_bytes, int *overflow_out);
 
+// Leaf sweep of a tree iteration (gsknn_sweep.hip).
+hipError_t launch_leaf_sweep_f32(int d, int k, const float *X, const float *X2, const int *perm,
+                                 const int *items, const int *nitems, float *D, int *I, int ldk,
+                                 int qpb, int ct, int grid, hipStream_t s);
+hipError_t launch_leaf_sweep_f64(int d, int k, const double *X, const double *X2, const int *perm,
+                                 const int *items, const int *nitems, double *D, int *I, int ldk,
+                                 int qpb, int ct, int grid, hipStream_t s);
+size_t leaf_sweep_lds_bytes(int elem, int d, int k, int qpb, int ct);
+
 // BF16 candidate pass + FP64 re-rank (gsknn_rerank.hip).
 size_t bf16_prep_bytes(int N, int d);
 hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_t s);

@@ -1,0 +1,234 @@
+// Device helpers shared by the kNN kernels: canonical order, exact
+// arithmetic, LDS heaps and the exact (oracle-arithmetic) block routine.
+//
+// Reference behaviour (ChenhanYu/rnn):
+//   distance  = (-2 * sum_p a_p b_p + XA2[amap[i]]) + XB2[bmap[j]], no clamp
+//               (frame/dgsknn_ref_impl.h:63-72, :91-103)
+//   selection = per heap row, keep the k smallest in a max-heap whose root is
+//               the current k-th distance (frame/gsknn_heap.c:104-162)
+#pragma once
+
+#include <limits.h>
+
+#include "gsknn_common.h"
+
+namespace gsknn {
+
+__host__ __device__ constexpr size_t al16(size_t x) { return (x + 15) & ~size_t(15); }
+
+template <typename T>
+__device__ __forceinline__ bool cless(T da, int ia, T db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// Rounded product and sum, never fused (the library is built with
+// -ffp-contract=off as well).
+__device__ __forceinline__ double xmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double xadd(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float xmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float xadd(float a, float b) { return __fadd_rn(a, b); }
+
+// Sift (xd, xi) down from slot s of the heap of query q (k entries, LDS,
+// stride QS between entries).
+template <typename T>
+__device__ void sift_down(T *hd, int *hi, int QS, int q, int s, int k, T xd, int xi) {
+  while (true) {
+    int c = 2 * s + 1;
+    if (c >= k) break;
+    T cd = hd[c * QS + q];
+    int ci = hi[c * QS + q];
+    if (c + 1 < k) {
+      T rd = hd[(c + 1) * QS + q];
+      int ri = hi[(c + 1) * QS + q];
+      if (cless(cd, ci, rd, ri)) {
+        ++c;
+        cd = rd;
+        ci = ri;
+      }
+    }
+    if (!cless(xd, xi, cd, ci)) break;
+    hd[s * QS + q] = cd;
+    hi[s * QS + q] = ci;
+    s = c;
+  }
+  hd[s * QS + q] = xd;
+  hi[s * QS + q] = xi;
+}
+
+template <typename T>
+__device__ __forceinline__ int heap_row(const KnnArgs<T> &a, int j) {
+  return a.hrow ? a.hrow[j] : j;
+}
+
+// Bring the resident row of query j into the LDS heap of slot q.  A launch
+// over all candidates heapifies the row under the canonical order; a split
+// launch starts from k copies of the row's canonical maximum instead, so only
+// candidates that beat the resident row end up in its partial list.
+template <typename T>
+__device__ void heap_load(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, int j,
+                          bool sentinel) {
+  const int k = a.k;
+  const size_t row = (size_t)heap_row(a, j) * a.ldk;
+  const T *Dr = a.D + row;
+  const int *Ir = a.I + row;
+  if (!sentinel) {
+    for (int t = 0; t < k; ++t) {
+      hd[t * QS + q] = Dr[t];
+      hi[t * QS + q] = Ir[t];
+    }
+    for (int s = k / 2 - 1; s >= 0; --s) sift_down(hd, hi, QS, q, s, k, hd[s * QS + q], hi[s * QS + q]);
+  } else {
+    T md = Dr[0];
+    int mi = Ir[0];
+    for (int t = 1; t < k; ++t) {
+      T x = Dr[t];
+      int xi = Ir[t];
+      if (cless(md, mi, x, xi)) {
+        md = x;
+        mi = xi;
+      }
+    }
+    for (int t = 0; t < k; ++t) {
+      hd[t * QS + q] = md;
+      hi[t * QS + q] = mi;
+    }
+  }
+}
+
+// Heapsort the LDS heap ascending in place, then write it out: descending
+// into the resident row, or ascending into the partial list of a split.
+template <typename T>
+__device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, int j, int split) {
+  const int k = a.k;
+  for (int e = k - 1; e > 0; --e) {
+    T xd = hd[e * QS + q];
+    int xi = hi[e * QS + q];
+    hd[e * QS + q] = hd[q];
+    hi[e * QS + q] = hi[q];
+    sift_down(hd, hi, QS, q, 0, e, xd, xi);
+  }
+  if (a.nsplit == 1) {
+    const size_t row = (size_t)heap_row(a, j) * a.ldk;
+    for (int t = 0; t < k; ++t) {
+      a.D[row + t] = hd[(k - 1 - t) * QS + q];
+      a.I[row + t] = hi[(k - 1 - t) * QS + q];
+    }
+  } else {
+    for (int t = 0; t < k; ++t) {
+      const size_t o = ((size_t)split * k + t) * a.n + j;
+      a.partD[o] = hd[t * QS + q];
+      a.partI[o] = hi[t * QS + q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact kernel: one thread per query, the reference oracle's arithmetic.
+// LDS: heap [k][QPB] | query rows [QPB][d+1] | candidate tile [d][CT] |
+//      candidate norms [CT] | candidate ids [CT]
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ExactLayout {
+  size_t hd, hi, sq, sc, cn, ci, bytes;
+  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT) {
+    hd = 0;
+    hi = al16(hd + sizeof(T) * (size_t)k * QPB);
+    sq = al16(hi + sizeof(int) * (size_t)k * QPB);
+    sc = al16(sq + sizeof(T) * (size_t)QPB * (d + 1));
+    cn = al16(sc + sizeof(T) * (size_t)d * CT);
+    ci = al16(cn + sizeof(T) * (size_t)CT);
+    bytes = al16(ci + sizeof(int) * (size_t)CT);
+  }
+};
+
+// One workgroup's share of a launch: queries [qblock*QPB, +QPB) against the
+// candidates of split `split`.  Every thread of the workgroup must call it.
+template <typename T, int QPB>
+__device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
+                            unsigned char *smem) {
+  const int k = a.k, d = a.d;
+  const ExactLayout<T> L(k, QPB, d, CT);
+  T *hd = (T *)(smem + L.hd);
+  int *hi = (int *)(smem + L.hi);
+  T *sq = (T *)(smem + L.sq);
+  T *sc = (T *)(smem + L.sc);
+  T *cn = (T *)(smem + L.cn);
+  int *ci = (int *)(smem + L.ci);
+
+  const int q = threadIdx.x;
+  const int j = qblock * QPB + q;
+  const bool valid = j < a.n;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+
+  T qn = T(0);
+  T rd = T(0);
+  int ri = 0;
+  T *myq = sq + (size_t)q * (d + 1);
+  if (valid) {
+    const T *xq = a.XB + (size_t)a.bmap[j] * d;
+    for (int p = 0; p < d; ++p) myq[p] = xq[p];
+    qn = a.XB2[a.bmap[j]];
+    heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
+    rd = hd[q];
+    ri = hi[q];
+  }
+
+  auto consider = [&](T s, int c) {
+    // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
+    const T dist = xadd(xadd(xmul(s, T(-2)), cn[c]), qn);
+    const int idx = ci[c];
+    if (cless(dist, idx, rd, ri)) {
+      sift_down(hd, hi, QPB, q, 0, k, dist, idx);
+      rd = hd[q];
+      ri = hi[q];
+    }
+  };
+
+  for (int c0 = cbeg; c0 < cend; c0 += CT) {
+    const int cnt = min(CT, cend - c0);
+    __syncthreads();
+    for (int e = q; e < cnt * d; e += QPB) {
+      const int c = e / d, p = e - c * d;
+      sc[p * CT + c] = a.XA[(size_t)a.amap[c0 + c] * d + p];
+    }
+    for (int e = q; e < cnt; e += QPB) {
+      const int id = a.amap[c0 + e];
+      cn[e] = a.XA2[id];
+      ci[e] = id;
+    }
+    __syncthreads();
+    if (!valid) continue;
+    int c = 0;
+    for (; c + 4 <= cnt; c += 4) {
+      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+      for (int p = 0; p < d; ++p) {
+        const T qp = myq[p];
+        const T *cp = sc + p * CT + c;
+        s0 = xadd(s0, xmul(cp[0], qp));
+        s1 = xadd(s1, xmul(cp[1], qp));
+        s2 = xadd(s2, xmul(cp[2], qp));
+        s3 = xadd(s3, xmul(cp[3], qp));
+      }
+      consider(s0, c);
+      consider(s1, c + 1);
+      consider(s2, c + 2);
+      consider(s3, c + 3);
+    }
+    for (; c < cnt; ++c) {
+      T s = T(0);
+      for (int p = 0; p < d; ++p) s = xadd(s, xmul(sc[p * CT + c], myq[p]));
+      consider(s, c);
+    }
+  }
+  if (valid) heap_store(a, hd, hi, QPB, q, j, split);
+}
+
+template <typename T, int QPB>
+__global__ __launch_bounds__(QPB) void knn_exact_kernel(KnnArgs<T> a, int CT) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  exact_block<T, QPB>(a, CT, blockIdx.x, blockIdx.y, smem);
+}
+
+
+}  // namespace gsknn

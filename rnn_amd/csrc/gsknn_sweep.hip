@@ -1,0 +1,91 @@
+// Batched all-nearest-neighbour sweep over the leaves of a randomized-
+// projection tree: one persistent launch per tree iteration.
+//
+// No counterpart in the reference; each leaf is the reference's gsknn call
+// with amap = bmap = the leaf's point ids and heap rows = the same ids
+// (merging into global heap rows, frame/gsknn.c:1138-1184 semantics), so the
+// result equals the composition of per-leaf dgsknn_ref/sgsknn_ref calls.
+//
+// Work list: items[i] = {leaf start in perm, leaf size, first query of the
+// block, 0}; the item count is read from device memory so that a launch
+// captured once in a hipGraph serves every iteration.  Each workgroup walks
+// the list with stride gridDim.x (items sorted by leaf size, largest first,
+// balance the persistent grid).  Within an item the exact block routine runs
+// one thread per query with the oracle's arithmetic, so distances are
+// bit-identical to the oracle's.  Leaves of one iteration are disjoint, so no
+// heap row is touched by two items of the same launch.
+#include "gsknn_device.h"
+
+namespace gsknn {
+
+template <typename T, int QPB>
+__global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, const int4 *items,
+                                                             const int *nitems, int CT) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int count = *nitems;
+  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+    const int4 w = items[it];
+    KnnArgs<T> a = base;
+    a.m = w.y;
+    a.n = w.y;
+    a.amap = base.amap + w.x;
+    a.bmap = base.amap + w.x;
+    a.hrow = base.amap + w.x;
+    a.nsplit = 1;
+    a.cps = w.y;
+    exact_block<T, QPB>(a, CT, w.z / QPB, 0, smem);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <typename T>
+size_t leaf_sweep_lds(int d, int k, int qpb, int ct) {
+  return ExactLayout<T>(k, qpb, d, ct).bytes;
+}
+
+template <typename T>
+hipError_t launch_leaf_sweep_t(int d, int k, const T *X, const T *X2, const int *perm,
+                               const int *items, const int *nitems, T *D, int *I, int ldk,
+                               int qpb, int ct, int grid, hipStream_t s) {
+  KnnArgs<T> a{};
+  a.d = d; a.k = k; a.ldk = ldk;
+  a.XA = X; a.XA2 = X2; a.XB = X; a.XB2 = X2;
+  a.amap = perm; a.bmap = perm;
+  a.D = D; a.I = I;
+  a.nsplit = 1;
+  const size_t lds = leaf_sweep_lds<T>(d, k, qpb, ct);
+  if (lds > 160 * 1024 || ct < 4 || (ct & 3) || grid < 1) return hipErrorNotSupported;
+  hipError_t e = hipSuccess;
+#define GSKNN_LEAF_CASE(QPB_)                                                             \
+  if (qpb == QPB_) {                                                                      \
+    auto kern = knn_leaf_sweep_kernel<T, QPB_>;                                           \
+    e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            (int)lds);                                                    \
+    if (e != hipSuccess) return e;                                                        \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(QPB_), lds, s, a, (const int4 *)items, nitems, ct); \
+    return hipGetLastError();                                                             \
+  }
+  GSKNN_LEAF_CASE(64)
+  GSKNN_LEAF_CASE(32)
+  GSKNN_LEAF_CASE(16)
+#undef GSKNN_LEAF_CASE
+  return hipErrorNotSupported;
+}
+
+hipError_t launch_leaf_sweep_f32(int d, int k, const float *X, const float *X2, const int *perm,
+                                 const int *items, const int *nitems, float *D, int *I, int ldk,
+                                 int qpb, int ct, int grid, hipStream_t s) {
+  return launch_leaf_sweep_t<float>(d, k, X, X2, perm, items, nitems, D, I, ldk, qpb, ct, grid, s);
+}
+
+hipError_t launch_leaf_sweep_f64(int d, int k, const double *X, const double *X2, const int *perm,
+                                 const int *items, const int *nitems, double *D, int *I, int ldk,
+                                 int qpb, int ct, int grid, hipStream_t s) {
+  return launch_leaf_sweep_t<double>(d, k, X, X2, perm, items, nitems, D, I, ldk, qpb, ct, grid, s);
+}
+
+size_t leaf_sweep_lds_bytes(int elem, int d, int k, int qpb, int ct) {
+  return elem == 8 ? leaf_sweep_lds<double>(d, k, qpb, ct) : leaf_sweep_lds<float>(d, k, qpb, ct);
+}
+
+}  // namespace gsknn

@@ -1,0 +1,95 @@
+"""RP-tree all-NN sweep: tree construction (CPU) and GPU parity with the
+composition of per-leaf oracle calls."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rnn_amd import datasets as ds
+from rnn_amd import sweep
+
+
+def _cpu_points(N=4096, d=8, seed=3):
+    import torch
+    return torch.from_numpy(ds.config4_points(N, d, 16, seed))
+
+
+def test_tree_partitions_points_into_bounded_leaves():
+    X = _cpu_points()
+    perm, leaves = sweep.build_tree(X, 0, seed=3, leaf_lo=64, leaf_hi=512)
+    p = perm.numpy()
+    assert np.array_equal(np.sort(p), np.arange(X.shape[0]))
+    starts = [s for s, _ in leaves]
+    sizes = [n for _, n in leaves]
+    assert starts[0] == 0 and sum(sizes) == X.shape[0]
+    assert all(s1 + n1 == s2 for (s1, n1), (s2, _) in zip(leaves, leaves[1:]))
+    assert max(sizes) <= 512 and min(sizes) >= 32
+    assert len(set(sizes)) > 1  # uneven leaves
+
+
+def test_tree_is_seeded_and_differs_across_iterations():
+    X = _cpu_points()
+    p0, l0 = sweep.build_tree(X, 0, seed=3, leaf_lo=64, leaf_hi=512)
+    p0b, l0b = sweep.build_tree(X, 0, seed=3, leaf_lo=64, leaf_hi=512)
+    p1, _ = sweep.build_tree(X, 1, seed=3, leaf_lo=64, leaf_hi=512)
+    assert np.array_equal(p0.numpy(), p0b.numpy()) and l0 == l0b
+    assert not np.array_equal(p0.numpy(), p1.numpy())
+
+
+def test_median_split_of_root():
+    # With one split forced (leaf_hi below N, leaf_lo above N/2), the two
+    # leaves are the lower and upper halves of the projection.
+    import torch
+    g = np.random.Generator(np.random.PCG64(1))
+    X = torch.from_numpy(g.standard_normal((200, 3)).astype(np.float32))
+    perm, leaves = sweep.build_tree(X, 0, seed=9, leaf_lo=150, leaf_hi=199)
+    assert leaves == [(0, 100), (100, 100)]
+    gg = np.random.Generator(np.random.PCG64([9, 0]))
+    gg.uniform(np.log(150), np.log(199), size=1)
+    direc = gg.standard_normal((1, 3))
+    direc /= np.linalg.norm(direc)
+    proj = (X.numpy() * direc.astype(np.float32)).sum(1)
+    left = set(perm.numpy()[:100])
+    assert max(proj[list(left)]) <= min(proj[list(set(range(200)) - left)])
+
+
+def test_items_cover_each_leaf_once():
+    leaves = [(0, 100), (100, 37), (137, 64)]
+    it = sweep.make_items(leaves, 32)
+    assert it[0][1] == 100  # largest first
+    cover = {}
+    for s, n, q0, _ in it:
+        cover.setdefault((s, n), []).append(q0)
+    assert sorted(cover[(0, 100)]) == [0, 32, 64, 96]
+    assert sorted(cover[(100, 37)]) == [0, 32]
+
+
+@pytest.mark.gpu
+def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
+    # Iteration 0 launches directly; iterations 1-3 replay the captured graph
+    # after eager tree builds.  The heap rows are checked after every
+    # iteration against the composition of per-leaf oracle calls.
+    import torch
+    monkeypatch.setenv("GSKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    dev = torch.device("cuda")
+    N, d, k = 1 << 14, 32, 16
+    Xn = ds.config4_points(N, d, 64, seed=3)
+    X = torch.from_numpy(Xn).to(dev)
+    sw = sweep.RPTreeSweep(X, k, seed=3, leaf_lo=64, leaf_hi=1024)
+    X2 = oracle.sqnorm(Xn)
+    assert np.array_equal(sw.X2.cpu().numpy(), X2)
+    Dr = np.full((N, k), np.inf, dtype=np.float32)
+    Ir = np.full((N, k), -1, dtype=np.int32)
+    for _ in range(4):
+        perm, leaves, info = sw.step()
+        print(f"[gpu] sweep iteration {info['iteration']}: {info['sweep_ms']:.3f} ms", flush=True)
+        perm = perm.cpu().numpy()
+        for s, n in leaves:
+            ids = np.ascontiguousarray(perm[s:s + n])
+            Ds, Is = np.ascontiguousarray(Dr[ids]), np.ascontiguousarray(Ir[ids])
+            oracle.gsknn(Xn, X2, ids, Xn, X2, ids, Ds, Is, canonical=True)
+            Dr[ids], Ir[ids] = Ds, Is
+        a = oracle.sort_rows(sw.D.cpu().numpy(), sw.I.cpu().numpy())
+        b = oracle.sort_rows(Dr, Ir)
+        np.testing.assert_array_equal(a[1], b[1])
+        assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
+    assert sw.graph is not None
