@@ -27,7 +27,7 @@ extern "C" {
  * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
  * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA.
  * FP32: AUTO/MFMA = FP32 MFMA candidate pass + re-rank in the oracle's FP32
- * arithmetic when d >= 8 and n >= 4096, else the exact VALU kernel; EXACT =
+ * arithmetic when d >= 8 and m >= 64, else the exact VALU kernel; EXACT =
  * always the VALU kernel.  Every FP32 mode returns the oracle's distance bits. */
 #define GSKNN_MODE_AUTO 0
 #define GSKNN_MODE_EXACT 1

@@ -84,6 +84,7 @@ __device__ void sift_fi(float *hd, int *hi, int s, int k, float xd, int xi) {
 
 struct F32Args {
   int m, n, d, k, ldk, cap;
+  int nsplit, cps;  // candidate splits (grid.y) and candidates per split
   const float *XA, *XA2;
   const int *amap;
   const float *XB, *XB2;
@@ -94,9 +95,9 @@ struct F32Args {
   float *xnA;   // [m] |XA[amap[i]]| rounded up
   float *xnB;   // [n] |XB[bmap[j]]| rounded up
   float *mx;    // {max xnA, max |XA2[amap[i]]|}
-  int2 *lst;    // [n][cap]
-  int *cnt;     // [n]
-  float *U;     // [n]
+  int2 *lst;    // [nsplit][n][cap]
+  int *cnt;     // [nsplit][n]
+  float *U;     // [nsplit][n]
   int *nover;
 };
 
@@ -163,6 +164,10 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4;
   const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const size_t sbase = (size_t)split * a.n;
 
   if (tid < Q) {
     const int j = qbase + tid;
@@ -214,12 +219,12 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
     dmx[qb] = (al[qb] * a.mx[0] + 4.0f * u * a.mx[1] + cq[qb]) * 1.002f + 1e-30f;
   }
 
-  const int ntiles = (a.m + C - 1) / C;
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   const int cs = tid / TPC;
   const int ch0 = (tid % TPC) * CPT;
   float4 stage[CPT];
   float st_n = 0.f, st_x = 0.f;
-  auto cand_pos = [&](int t) { return min(t * C + cs, a.m - 1); };
+  auto cand_pos = [&](int t) { return min(cbeg + t * C + cs, cend - 1); };
   auto load_x = [&](int pos, int id) {
     const float *row = a.XA + (size_t)id * d;
 #pragma unroll
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
     st_x = a.xnA[pos];
   };
   auto store_x = [&](int t) {
-    const bool ok = t * C + cs < a.m;
+    const bool ok = cbeg + t * C + cs < cend;
     const int cb = cs >> 4, i = cs & 15;
 #pragma unroll
     for (int e = 0; e < CPT; ++e) {
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
     const float Uq = hv[slot];
     if (lower <= Uq) {
       const int at = hc[slot]++;
-      if (at < a.cap) a.lst[(size_t)(qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+      if (at < a.cap) a.lst[(sbase + qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
     }
     if (upper < Uq) sift_v(hv, Q, slot, k, upper);
   };
@@ -366,18 +371,18 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
               for (int bb = 1; bb < CB * 4; ++bb)
                 if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
               const int c = (b >> 2) * 16 + 4 * grp + (b & 3);
-              if (t * C + c >= a.m) continue;
+              if (cbeg + t * C + c >= cend) continue;
               const float dist = tt + nq[qb];
               const float delta = txn[c] * al[qb] + 4.0f * u * fabsf(tnc[c]) + cq[qb] + 1e-30f;
               if (direct) {
-                apply(ql[qb], dist - delta, dist + delta, t * C + c);
+                apply(ql[qb], dist - delta, dist + delta, cbeg + t * C + c);
               } else {
 #pragma unroll
                 for (int r = 0; r < QR; ++r)
                   if (pq_n == r) {
                     pq_lo[r] = dist - delta;
                     pq_up[r] = dist + delta;
-                    pq_ps[r] = t * C + c;
+                    pq_ps[r] = cbeg + t * C + c;
                     pq_q[r] = qb;
                   }
                 ++pq_n;
@@ -404,8 +409,8 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
   if (tid < Q) {
     const int j = qbase + tid;
     if (j < a.n) {
-      a.U[j] = hv[tid];
-      a.cnt[j] = hc[tid];
+      a.U[sbase + j] = hv[tid];
+      a.cnt[sbase + j] = hc[tid];
       if (hc[tid] > a.cap) atomicAdd(a.nover, 1);
     }
   }
@@ -443,40 +448,45 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
   }
   __builtin_amdgcn_wave_barrier();
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  const float Ulim = a.U[j];
-  const int cnt = a.cnt[j];
-  const bool over = cnt > a.cap;
-  const int nscan = over ? a.m : cnt;
-  for (int e0 = 0; e0 < nscan; e0 += 64) {
-    const int e = e0 + lane;
-    bool have = false;
-    int pos = 0;
-    if (e < nscan) {
-      if (over) {
-        pos = e;
-        have = true;
-      } else {
-        const int2 en = a.lst[(size_t)j * a.cap + e];
-        pos = en.x;
-        have = __int_as_float(en.y) <= Ulim;
+  float Ulim = a.U[j];
+  for (int sp = 1; sp < a.nsplit; ++sp) Ulim = fminf(Ulim, a.U[(size_t)sp * a.n + j]);
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const size_t sj = (size_t)sp * a.n + j;
+    const int cnt = a.cnt[sj];
+    const bool over = cnt > a.cap;
+    const int cbeg = sp * a.cps;
+    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : cnt;
+    for (int e0 = 0; e0 < nscan; e0 += 64) {
+      const int e = e0 + lane;
+      bool have = false;
+      int pos = 0;
+      if (e < nscan) {
+        if (over) {
+          pos = cbeg + e;
+          have = true;
+        } else {
+          const int2 en = a.lst[sj * a.cap + e];
+          pos = en.x;
+          have = __int_as_float(en.y) <= Ulim;
+        }
       }
-    }
-    float dist = 0.0f;
-    int id = 0;
-    if (have) {
-      dist = oracle_dist_f32(a, pos, j);
-      id = a.amap[pos];
-    }
-    // Merge in lane order; only lanes that may beat the root take a turn.
-    unsigned long long todo = __ballot(have && cl_f(dist, id, hd[0], hi[0]));
-    while (todo) {
-      const int src = __builtin_ctzll(todo);
-      todo &= todo - 1;
-      const float xd = __shfl(dist, src, 64);
-      const int xi = __shfl(id, src, 64);
-      if (lane == 0 && cl_f(xd, xi, hd[0], hi[0])) sift_fi(hd, hi, 0, k, xd, xi);
-      __builtin_amdgcn_wave_barrier();
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      float dist = 0.0f;
+      int id = 0;
+      if (have) {
+        dist = oracle_dist_f32(a, pos, j);
+        id = a.amap[pos];
+      }
+      // Merge in lane order; only lanes that may beat the root take a turn.
+      unsigned long long todo = __ballot(have && cl_f(dist, id, hd[0], hi[0]));
+      while (todo) {
+        const int src = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const float xd = __shfl(dist, src, 64);
+        const int xi = __shfl(id, src, 64);
+        if (lane == 0 && cl_f(xd, xi, hd[0], hi[0])) sift_fi(hd, hi, 0, k, xd, xi);
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      }
     }
   }
   if (lane == 0) {
@@ -499,9 +509,27 @@ namespace {
 constexpr size_t kLdsF = 160 * 1024;
 
 struct F32Plan {
-  int ks = 0, qb = 0, cb = 0, w = 0, q = 0, cap = 0;
+  int ks = 0, qb = 0, cb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
   size_t lds = 0;
 };
+
+// Split the candidates when the query blocks alone give fewer than two
+// workgroups per CU; each split keeps at least 16 tiles.
+void plan_f32_splits(F32Plan &p, int m, int n) {
+  const int gx = (n + p.q - 1) / p.q;
+  const int C = 16 * p.cb;
+  int ns = 1;
+  if (gx < 512) {
+    ns = (512 + gx - 1) / gx;
+    const int maxs = m / (16 * C);
+    if (ns > maxs) ns = maxs;
+    if (ns < 1) ns = 1;
+  }
+  int cps = (m + ns - 1) / ns;
+  cps = (cps + C - 1) / C * C;
+  p.nsplit = (m + cps - 1) / cps;
+  p.cps = cps;
+}
 
 F32Plan plan_f32(int d, int k) {
   F32Plan p;
@@ -533,21 +561,22 @@ F32Plan plan_f32(int d, int k) {
 
 struct F32Work {
   size_t xnA, xnB, mx, lst, cnt, U, nover, bytes;
-  F32Work(int m, int n, int cap) {
+  F32Work(int m, int n, int cap, int nsplit) {
+    const size_t rows = (size_t)n * nsplit;
     xnA = 0;
     xnB = a16(xnA + (size_t)m * 4);
     mx = a16(xnB + (size_t)n * 4);
     lst = a16(mx + 16);
-    cnt = a16(lst + (size_t)n * cap * 8);
-    U = a16(cnt + (size_t)n * 4);
-    nover = a16(U + (size_t)n * 4);
+    cnt = a16(lst + rows * cap * 8);
+    U = a16(cnt + rows * 4);
+    nover = a16(U + rows * 4);
     bytes = a16(nover + 16);
   }
 };
 
 template <int KS>
 hipError_t launch_f32_ks(const F32Plan &p, const F32Args &a, hipStream_t s) {
-  const dim3 grid((a.n + p.q - 1) / p.q);
+  const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
   constexpr int CBk = KS >= 32 ? 1 : 2;
 #define GSKNN_F32_CASE(QB_, W_)                                                        \
   if (p.qb == QB_ && p.w == W_) {                                                      \
@@ -570,9 +599,10 @@ hipError_t launch_f32_ks(const F32Plan &p, const F32Args &a, hipStream_t s) {
 }  // namespace
 
 size_t f32_work_bytes(int m, int n, int d, int k) {
-  const F32Plan p = plan_f32(d, k);
+  F32Plan p = plan_f32(d, k);
   if (p.q == 0 || m <= 0 || n <= 0) return 0;
-  return F32Work(m, n, p.cap).bytes;
+  plan_f32_splits(p, m, n);
+  return F32Work(m, n, p.cap, p.nsplit).bytes;
 }
 
 bool f32_mfma_supported(int d, int k) {
@@ -583,15 +613,17 @@ bool f32_mfma_supported(int d, int k) {
 
 hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *work,
                                size_t work_bytes, int *overflow_out) {
-  const F32Plan p = plan_f32(ka.d, ka.k);
+  F32Plan p = plan_f32(ka.d, ka.k);
   if (p.q == 0) return hipErrorNotSupported;
+  plan_f32_splits(p, ka.m, ka.n);
   const size_t rl = a16(sizeof(float) * 4 * (size_t)ka.k) + sizeof(int) * 4 * (size_t)ka.k;
   if (rl > kLdsF) return hipErrorNotSupported;
-  const F32Work W(ka.m, ka.n, p.cap);
+  const F32Work W(ka.m, ka.n, p.cap, p.nsplit);
   if (work == nullptr || work_bytes < W.bytes) return hipErrorInvalidValue;
   char *wb = (char *)work;
   F32Args a{};
   a.m = ka.m; a.n = ka.n; a.d = ka.d; a.k = ka.k; a.ldk = ka.ldk; a.cap = p.cap;
+  a.nsplit = p.nsplit; a.cps = p.cps;
   a.XA = ka.XA; a.XA2 = ka.XA2; a.amap = ka.amap;
   a.XB = ka.XB; a.XB2 = ka.XB2; a.bmap = ka.bmap;
   a.D = ka.D; a.I = ka.I; a.hrow = ka.hrow;

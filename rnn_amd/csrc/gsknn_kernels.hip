@@ -575,17 +575,17 @@ size_t part_bytes(const Plan &p, int n, int k, int elem) {
   return al16(ents * elem) + al16(ents * sizeof(int));
 }
 
-// FP32 calls with enough queries to fill the chip and d >= 8 take the FP32
-// MFMA candidate pass + oracle-arithmetic re-rank; the rest the exact kernel.
-bool use_f32_mfma(int n, int d, int k, int mode) {
-  return mode != MODE_EXACT && d >= 8 && n >= 4096 && f32_mfma_supported(d, k);
+// FP32 calls with d >= 8 and at least 64 candidates take the FP32 MFMA
+// candidate pass + oracle-arithmetic re-rank; the rest the exact kernel.
+bool use_f32_mfma(int m, int d, int k, int mode) {
+  return mode != MODE_EXACT && d >= 8 && m >= 64 && f32_mfma_supported(d, k);
 }
 
 template <typename T>
 hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t work_bytes) {
   if (a.m <= 0 || a.n <= 0 || a.d <= 0 || a.k <= 0) return hipSuccess;
   if constexpr (sizeof(T) == 4) {
-    if (use_f32_mfma(a.n, a.d, a.k, mode)) return launch_knn_f32_mfma(a, s, work, work_bytes, nullptr);
+    if (use_f32_mfma(a.m, a.d, a.k, mode)) return launch_knn_f32_mfma(a, s, work, work_bytes, nullptr);
   }
   const Plan p = make_plan(a.m, a.n, a.d, a.k, (int)sizeof(T), mode);
   if (p.qpb == 0) return hipErrorNotSupported;
@@ -624,7 +624,7 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
 
 size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
   if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return 0;
-  if (elem_bytes == 4 && use_f32_mfma(n, d, k, mode)) return f32_work_bytes(m, n, d, k);
+  if (elem_bytes == 4 && use_f32_mfma(m, d, k, mode)) return f32_work_bytes(m, n, d, k);
   const Plan p = make_plan(m, n, d, k, elem_bytes, mode);
   return part_bytes(p, n, k, elem_bytes);
 }

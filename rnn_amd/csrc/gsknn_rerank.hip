@@ -119,6 +119,7 @@ hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_
 
 struct Bf16Args {
   int m, n, d, k, ldk, cap;
+  int nsplit, cps;             // candidate splits (grid.y) and candidates per split
   int NA, NB;                  // rows of the candidate / query tables
   const float *XA, *XB;        // FP32 tables (re-rank)
   const unsigned char *prepA, *prepB;
@@ -126,9 +127,9 @@ struct Bf16Args {
   double *D;
   int *I;
   const int *hrow;
-  int2 *lst;    // [n][cap] (candidate position, lower bound bits)
-  int *cnt;     // [n]
-  float *U;     // [n]
+  int2 *lst;    // [nsplit][n][cap] (candidate position, lower bound bits)
+  int *cnt;     // [nsplit][n]
+  float *U;     // [nsplit][n]
   int *nover;   // queries whose list overflowed
 };
 
@@ -217,6 +218,10 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4;
   const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const size_t sbase = (size_t)split * a.n;  // this split's rows of lst/cnt/U
 
   // Upper-bound heaps start from the resident rows (rounded up to FP32).
   if (tid < Q) {
@@ -269,12 +274,12 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f;
   }
 
-  const int ntiles = (a.m + C - 1) / C;
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   const int cs = tid / TPC;
   const int ch0 = (tid % TPC) * CPT;
   uint4 stage[CPT];
   float st_n = 0.f, st_x = 0.f, st_e = 0.f;
-  auto cand_pos = [&](int t) { return min(t * C + cs, a.m - 1); };
+  auto cand_pos = [&](int t) { return min(cbeg + t * C + cs, cend - 1); };
   auto load_x = [&](int id) {
     const uint4 *row = (const uint4 *)(xbA + (size_t)id * DP);
 #pragma unroll
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     st_e = enA[id];
   };
   auto store_x = [&](int t) {
-    const bool ok = t * C + cs < a.m;
+    const bool ok = cbeg + t * C + cs < cend;
     const int cb = cs >> 4, i = cs & 15;
 #pragma unroll
     for (int e = 0; e < CPT; ++e) {
@@ -335,7 +340,7 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     if (lower <= Uq) {
       const int at = hc[slot]++;
       if (at < a.cap)
-        a.lst[(size_t)(qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+        a.lst[(sbase + qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
     }
     if (upper < Uq) sift_vals(hv, Q, slot, k, upper);
   };
@@ -417,11 +422,11 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
                 for (int bb = 1; bb < CB * 4; ++bb)
                   if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
                 const int c = (b >> 2) * 16 + out_row(lane, b & 3);
-                if (t * C + c >= a.m) continue;
+                if (cbeg + t * C + c >= cend) continue;
                 const float dist = tt + nq[qb];
                 const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
                                         1.0009765625f + 1e-30f;
-                apply(ql[qb], dist - delta, dist + delta, t * C + c);
+                apply(ql[qb], dist - delta, dist + delta, cbeg + t * C + c);
               }
             }
           }
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
             for (int bb = 1; bb < CB * 4; ++bb)
               if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
             const int c = (b >> 2) * 16 + out_row(lane, b & 3);
-            if (t * C + c >= a.m) continue;
+            if (cbeg + t * C + c >= cend) continue;
             const float dist = tt + nq[qb];
             const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
                                     1.0009765625f + 1e-30f;
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
               if (pq_n == r) {
                 pq_lo[r] = dist - delta;
                 pq_up[r] = dist + delta;
-                pq_ps[r] = t * C + c;
+                pq_ps[r] = cbeg + t * C + c;
                 pq_q[r] = qb;
               }
             ++pq_n;
@@ -471,8 +476,8 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
   if (tid < Q) {
     const int j = qbase + tid;
     if (j < a.n) {
-      a.U[j] = hv[tid];
-      a.cnt[j] = hc[tid];
+      a.U[sbase + j] = hv[tid];
+      a.cnt[sbase + j] = hc[tid];
       if (hc[tid] > a.cap) atomicAdd(a.nover, 1);
     }
   }
@@ -528,36 +533,42 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const float *xq = a.XB + (size_t)a.bmap[j] * a.d;
-  const float Uf = a.U[j];
+  // Every split's U bounds the final k-th distance; prune with the smallest.
+  float Uf = a.U[j];
+  for (int sp = 1; sp < a.nsplit; ++sp) Uf = fminf(Uf, a.U[(size_t)sp * a.n + j]);
   const float Ulim = Uf * (1.0f + 0x1p-20f) + 1e-30f;
-  const int cnt = a.cnt[j];
-  const bool over = cnt > a.cap;
-  const int nscan = over ? a.m : cnt;
   double rd = hd[0];
   int ri = hi[0];
-  for (int e = 0; e < nscan; ++e) {
-    int pos;
-    if (over) {
-      pos = e;
-    } else {
-      const int2 en = a.lst[(size_t)j * a.cap + e];
-      if (__int_as_float(en.y) > Ulim) continue;
-      pos = en.x;
-    }
-    const int id = a.amap[pos];
-    const float *xa = a.XA + (size_t)id * a.d;
-    double s = 0.0;
-    for (int p = lane; p < a.d; p += 64) {
-      const double t = (double)xa[p] - (double)xq[p];
-      s = __builtin_fma(t, t, s);
-    }
-    const double dist = wave_sum(s);
-    if (cless_d(dist, id, rd, ri)) {
-      if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, id);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      rd = hd[0];
-      ri = hi[0];
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const size_t sj = (size_t)sp * a.n + j;
+    const int cnt = a.cnt[sj];
+    const bool over = cnt > a.cap;
+    const int cbeg = sp * a.cps;
+    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : cnt;
+    for (int e = 0; e < nscan; ++e) {
+      int pos;
+      if (over) {
+        pos = cbeg + e;
+      } else {
+        const int2 en = a.lst[sj * a.cap + e];
+        if (__int_as_float(en.y) > Ulim) continue;
+        pos = en.x;
+      }
+      const int id = a.amap[pos];
+      const float *xa = a.XA + (size_t)id * a.d;
+      double s = 0.0;
+      for (int p = lane; p < a.d; p += 64) {
+        const double t = (double)xa[p] - (double)xq[p];
+        s = __builtin_fma(t, t, s);
+      }
+      const double dist = wave_sum(s);
+      if (cless_d(dist, id, rd, ri)) {
+        if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, id);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        rd = hd[0];
+        ri = hi[0];
+      }
     }
   }
   if (lane == 0) {
@@ -581,9 +592,27 @@ constexpr size_t kLds = 160 * 1024;
 constexpr int kCBb = 4;
 
 struct Bf16Plan {
-  int ks = 0, qb = 0, w = 0, q = 0, cap = 0;
+  int ks = 0, qb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
   size_t lds = 0;
 };
+
+// Split the candidates when the query blocks alone give fewer than two
+// workgroups per CU; each split keeps at least 16 tiles.
+void plan_splits(Bf16Plan &p, int m, int n) {
+  const int gx = (n + p.q - 1) / p.q;
+  const int C = 16 * kCBb;
+  int ns = 1;
+  if (gx < 512) {
+    ns = (512 + gx - 1) / gx;
+    const int maxs = m / (16 * C);
+    if (ns > maxs) ns = maxs;
+    if (ns < 1) ns = 1;
+  }
+  int cps = (m + ns - 1) / ns;
+  cps = (cps + C - 1) / C * C;
+  p.nsplit = (m + cps - 1) / cps;
+  p.cps = cps;
+}
 
 Bf16Plan plan_bf16(int d, int k) {
   Bf16Plan p;
@@ -612,14 +641,16 @@ Bf16Plan plan_bf16(int d, int k) {
   return p;
 }
 
-size_t bf16_work_layout(int n, int cap, size_t *o_lst, size_t *o_cnt, size_t *o_u, size_t *o_over) {
+size_t bf16_work_layout(int n, int cap, int nsplit, size_t *o_lst, size_t *o_cnt, size_t *o_u,
+                        size_t *o_over) {
   size_t off = 0;
+  const size_t rows = (size_t)n * nsplit;
   *o_lst = off;
-  off = al16r(off + (size_t)n * cap * sizeof(int2));
+  off = al16r(off + rows * cap * sizeof(int2));
   *o_cnt = off;
-  off = al16r(off + (size_t)n * 4);
+  off = al16r(off + rows * 4);
   *o_u = off;
-  off = al16r(off + (size_t)n * 4);
+  off = al16r(off + rows * 4);
   *o_over = off;
   off = al16r(off + 16);
   return off;
@@ -627,7 +658,7 @@ size_t bf16_work_layout(int n, int cap, size_t *o_lst, size_t *o_cnt, size_t *o_
 
 template <int KS>
 hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s) {
-  const dim3 grid((a.n + p.q - 1) / p.q);
+  const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
 #define GSKNN_BF16_CASE(QB_, W_)                                                       \
   if (p.qb == QB_ && p.w == W_) {                                                      \
     auto kern = knn_bf16_filter_kernel<KS, QB_, kCBb, W_>;                              \
@@ -649,11 +680,11 @@ hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s)
 }  // namespace
 
 size_t bf16_work_bytes(int m, int n, int d, int k) {
-  (void)m;
-  const Bf16Plan p = plan_bf16(d, k);
-  if (p.ks == 0 || p.q == 0 || n <= 0) return 0;
+  Bf16Plan p = plan_bf16(d, k);
+  if (p.ks == 0 || p.q == 0 || n <= 0 || m <= 0) return 0;
+  plan_splits(p, m, n);
   size_t a, b, c, e;
-  return bf16_work_layout(n, p.cap, &a, &b, &c, &e);
+  return bf16_work_layout(n, p.cap, p.nsplit, &a, &b, &c, &e);
 }
 
 hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const float *XA,
@@ -661,15 +692,17 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
                            const int *bmap, double *D, int *I, int ldk, const int *hrow,
                            void *work, size_t work_bytes, int *overflow_out, hipStream_t s) {
   if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return hipSuccess;
-  const Bf16Plan p = plan_bf16(d, k);
+  Bf16Plan p = plan_bf16(d, k);
   if (p.ks == 0 || p.q == 0) return hipErrorNotSupported;
+  plan_splits(p, m, n);
   const size_t rl = al16r(sizeof(double) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;
   if (rl > kLds) return hipErrorNotSupported;
   size_t o_lst, o_cnt, o_u, o_over;
-  const size_t need = bf16_work_layout(n, p.cap, &o_lst, &o_cnt, &o_u, &o_over);
+  const size_t need = bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over);
   if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
   Bf16Args a{};
   a.m = m; a.n = n; a.d = d; a.k = k; a.ldk = ldk; a.cap = p.cap;
+  a.nsplit = p.nsplit; a.cps = p.cps;
   a.NA = NA; a.NB = NB;
   a.XA = XA; a.XB = XB;
   a.prepA = (const unsigned char *)prepA;

@@ -379,3 +379,34 @@ def test_sgsknn_fp32_mfma_rerank_bitwise(d, k, lattice):
     Dc, Ic = D.copy(), I.copy()
     oracle.gsknn(X, X2, amap, X, X2, bmap, Dc, Ic, canonical=False)
     assert np.array_equal(_sorted(Dg, Ig)[0], _sorted(Dc, Ic)[0])
+
+
+def test_bf16_and_fp32_few_queries_many_candidates():
+    # Few queries against many candidates: the filter kernels split the
+    # candidates over workgroups (per-split bounds and lists).
+    import torch
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(99))
+    X = ds.low_rank(60000, 64, 8, seed=5)
+    amap = np.arange(60000, dtype=np.int32)
+    bmap = g.permutation(60000)[:37].astype(np.int32)
+    k = 24
+    print("[gpu] split bf16/fp32 n=37 m=60000", flush=True)
+    D, I = ds.empty_heap(bmap.size, k)
+    Dg, Ig, over = _bf16_run(X, amap, X, bmap, D.copy(), I.copy())
+    X64 = X.astype(np.float64)
+    X2 = oracle.sqnorm(X64)
+    Dr, Ir = D.copy(), I.copy()
+    oracle.gsknn(X64, X2, amap, X64, X2, bmap, Dr, Ir, canonical=True)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+    X2f = oracle.sqnorm(X)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2f).to(dev)
+    Df, If = ds.empty_heap(bmap.size, k, np.float32)
+    tD, tI = torch.from_numpy(Df).to(dev), torch.from_numpy(If).to(dev)
+    gsknn.sknn_dev(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2,
+                   torch.from_numpy(bmap).to(dev), tD, tI)
+    torch.cuda.synchronize()
+    Dr, Ir = Df.copy(), If.copy()
+    oracle.gsknn(X, X2f, amap, X, X2f, bmap, Dr, Ir, canonical=True)
+    assert_rows(tD.cpu().numpy(), tI.cpu().numpy(), Dr, Ir)
