@@ -238,12 +238,61 @@ def allnn_sweep(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+def serving(torch, dev, args):
+    """Config 5: device-resident reference set of 2,000,000 x 128 FP32 points
+    (low-rank generator, latent 16); Poisson arrivals at 200 req/s; request
+    sizes log-uniform in [1, 8192] queries; k uniform in {10, 32, 100}.
+    Continuous batching; p50/p99 of (completion - arrival) and pairs/s for the
+    BF16 + FP64 re-rank form and for plain FP32, plus their index agreement."""
+    import numpy as np
+
+    from rnn_amd import datasets as ds
+    from rnn_amd import serve
+
+    N = 2_000_000 // args.scale
+    t0 = time.perf_counter()
+    Rn, W = ds.config5_reference(N, 128)
+    R = torch.from_numpy(Rn).to(dev)
+    base = serve.make_trace(W, args.requests, rate=args.rate, seed=5, d=128)
+    nq = sum(r.queries.shape[0] for r in base)
+    log(f"c5 data N={N} requests={len(base)} queries={nq} ({time.perf_counter() - t0:.1f}s)")
+    out = {"workload": "c5", "N": N, "d": 128, "requests": len(base), "queries": nq,
+           "rate_req_per_s": args.rate, "offered_pairs_per_s": nq * N / base[-1].arrival}
+    results = {}
+    for mode in ("bf16", "fp32"):
+        srv = serve.KnnServer(R, mode=mode, hbm_budget=32 << 30, max_batch=16384, streams=3)
+        warm = [serve.Request(-1, 0.0, base[0].queries[:8], kk) for kk in (10, 32, 100)]
+        srv.run(warm, realtime=False)
+        srv.batch_sizes.clear()
+        srv.batches_launched = 0
+        reqs = [serve.Request(r.rid, r.arrival, r.queries, r.k) for r in base]
+        wall, done = srv.run(reqs, realtime=True)
+        st = serve.latency_stats(done)
+        st.update({"wall_s": wall, "pairs_per_s": nq * float(N) / wall,
+                   "batches": srv.batches_launched,
+                   "mean_batch_queries": float(np.mean(srv.batch_sizes))})
+        out[mode] = st
+        results[mode] = {r.rid: r.I.numpy() for r in done}
+        log(f"{mode}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms, "
+            f"{st['pairs_per_s']:.3e} pairs/s, {st['batches']} batches")
+    same = 0
+    rows = 0
+    for rid, Ib in results["bf16"].items():
+        If = results["fp32"][rid]
+        same += int(np.sum(np.all(np.sort(Ib, 1) == np.sort(If, 1), axis=1)))
+        rows += Ib.shape[0]
+    out["rows_equal_bf16_vs_fp32"] = same / max(rows, 1)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "copy"])
+    ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
+    ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
     ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -269,6 +318,9 @@ def main():
         return
     if args.workload == "c4":
         allnn_sweep(torch, dev, args)
+        return
+    if args.workload == "c5":
+        serving(torch, dev, args)
         return
     t0 = time.perf_counter()
     p = ds.config2()
