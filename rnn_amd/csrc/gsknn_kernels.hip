@@ -26,6 +26,12 @@
 // of different queries hit different banks.  Rows are written back sorted
 // in descending order, which is a valid max-heap.
 #include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <atomic>
+#include <type_traits>
 
 #include "gsknn_device.h"
 
@@ -35,13 +41,20 @@ namespace gsknn {
 // FP64 MFMA kernel.
 //
 // A workgroup of W waves owns Q = W * 16 * QB queries; wave w owns 16 * QB of
-// them.  Candidates stream through LDS in tiles of C = 16 * CB, double
-// buffered, stored in MFMA fragment order so that each A-operand read is 64
-// consecutive doubles.  v_mfma_f64_16x16x4_f64 takes A = 16 candidates x 4
-// dims (lane l: candidate l%16, dim l/16) and B = 4 dims x 16 queries (lane l:
-// dim l/16, query l%16); lane l of the 16x16 result holds query l%16 against
-// candidates l/16 + 4*v, v = 0..3 (measured on gfx950).  The query fragments
-// stay in registers for the whole launch (d <= 4*KS).
+// them.  Candidates stream through LDS in tiles of C = 16 * CB, stored in
+// MFMA fragment order so that each A-operand read is 64 consecutive doubles,
+// with the tile's candidate norms and ids beside it.
+// v_mfma_f64_16x16x4_f64 takes A = 16 candidates x 4 dims (lane l: candidate
+// l%16, dim l/16) and B = 4 dims x 16 queries (lane l: dim l/16, query l%16);
+// lane l of the 16x16 result holds query l%16 against candidates l/16 + 4*v,
+// v = 0..3 (measured on gfx950).  The query fragments stay in registers for
+// the whole launch (d <= 4*KS).
+//
+// NB = 2 tile buffers: the next tile is written while the current one is
+// read, one barrier per tile; lane l of every wave holds the norm and id of
+// tile candidate l % C in registers (loaded a tile ahead) and the epilogue
+// fetches them with lane shuffles.  NB = 1: two barriers per tile, norms and
+// ids in LDS.
 //
 // Epilogue per lane: dist = (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]], test
 // against the row's root.  A wave-wide ballot skips the insertion step when
@@ -51,13 +64,16 @@ namespace gsknn {
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// NB = 1 keeps the tile's candidate norms and ids in LDS beside the tile;
+// NB = 2 keeps them in registers (every LDS byte goes to the two tiles and
+// the heaps, which keeps two 128-query workgroups per CU at k = 32, d = 64).
 struct MfmaLayout {
   size_t tA, tn, ti, hd, hi, bytes;
-  __host__ __device__ MfmaLayout(int KS, int CB, int Q, int k) {
+  __host__ __device__ MfmaLayout(int KS, int CB, int Q, int k, int NB) {
     const size_t tile = (size_t)KS * CB * 64;
-    const size_t C = 16 * (size_t)CB;
+    const size_t C = NB == 1 ? 16 * (size_t)CB : 0;
     tA = 0;
-    tn = al16(tA + sizeof(double) * tile);
+    tn = al16(tA + sizeof(double) * tile * NB);
     ti = al16(tn + sizeof(double) * C);
     hd = al16(ti + sizeof(int) * C);
     hi = al16(hd + sizeof(double) * (size_t)k * Q);
@@ -65,22 +81,32 @@ struct MfmaLayout {
   }
 };
 
-// Candidate tiles go global -> registers -> LDS: thread t stages dims
-// [p0, p0 + DPT) of candidate slot t / TPC.  The amap entries are read two
-// tiles ahead and the point rows one tile ahead, so no load waits on another
-// inside the loop and the loads overlap the MFMA work of the current tile.
-// One LDS tile buffer (two barriers per tile) keeps the C=32, k=32 workgroup
-// at 64 KB so that two workgroups share a CU.
-template <int KS, int QB, int CB, int W>
-__global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a) {
-  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, DP = 4 * KS, NT = 64 * W;
-  static_assert(NT % C == 0, "threads per candidate");
-  constexpr int TPC = NT / C;
-  static_assert(DP % TPC == 0, "dims per thread");
-  constexpr int DPT = DP / TPC;
+// Candidate tiles go global -> registers -> LDS.  The amap entries are read
+// two tiles ahead and the point rows one tile ahead, so no load waits on
+// another inside the loop and the loads overlap the MFMA work of the
+// current tile.
+// Two waves per SIMD whenever the query fragments (2 * QB * KS registers)
+// and the staged tile elements (KS * CB / W doubles) leave room: the
+// compiler then keeps the accumulators in VGPRs and fits the survivor queue
+// below 256 registers.
+constexpr int mfma_waves_per_eu(int KS, int QB, int CB, int W) {
+  return (QB * KS <= 32 && KS * CB <= 8 * W) ? 2 : 1;
+}
+
+template <int KS, int QB, int CB, int W, int NB>
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_mfma_f64_kernel(
+    KnnArgs<double> a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
+  static_assert(NB == 1 || NB == 2, "tile buffers");
+  static_assert(C <= 64, "one candidate per lane");
+  constexpr int TILE = KS * CB * 64;
+  // Candidate ids of a lane's CB * 4 result values; a vector type keeps the
+  // per-bit selection in registers.
+  typedef int idv __attribute__((ext_vector_type(CB * 4)));
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k, d = a.d;
-  const MfmaLayout L(KS, CB, Q, k);
+  const MfmaLayout L(KS, CB, Q, k, NB);
   double *tA = (double *)(smem + L.tA);
   double *tn = (double *)(smem + L.tn);
   int *ti = (int *)(smem + L.ti);
@@ -118,93 +144,158 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
   }
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
-  const int cs = tid / TPC;
-  const int p0 = (tid % TPC) * DPT;
-  double stage[DPT];
+  // Staging: the tile is BLK fragment blocks of 64 doubles (k-step s,
+  // candidate block cb); wave w stages blocks w * BPW .. w * BPW + BPW - 1 and
+  // lane l writes element l of each, i.e. dim 4 s + l / 16 of candidate
+  // cb * 16 + l % 16.  Every LDS store is then 512 contiguous bytes (no bank
+  // conflicts) and each lane gathers 8-byte elements, 32 contiguous bytes
+  // per candidate row and instruction.  A thread touches the candidates
+  // cb * 16 + l % 16 for cb < CB; their ids travel in an Ids.
+  constexpr int BLK = KS * CB;
+  static_assert(BLK % W == 0, "fragment blocks per wave");
+  constexpr int BPW = BLK / W;
+  const int si = lane & 15, skk = lane >> 4;
+  struct Ids {
+    int v[CB];
+  };
+  double stage[BPW];
   double stage_n = 0.0;
-  auto cand_id = [&](int t) -> int {
-    const int c = min(cbeg + t * C + cs, cend - 1);
-    return a.amap[c];
-  };
-  auto load_x = [&](int id) {
-    const double *row = a.XA + (size_t)id * d;
+  auto cand_id = [&](int t) -> Ids {
+    Ids r;
 #pragma unroll
-    for (int e = 0; e < DPT; ++e) {
-      const int p = p0 + e;
-      stage[e] = row[p < d ? p : d - 1];
-    }
-    stage_n = a.XA2[id];
+    for (int cb = 0; cb < CB; ++cb) r.v[cb] = a.amap[min(cbeg + t * C + cb * 16 + si, cend - 1)];
+    return r;
   };
-  auto store_x = [&](int t, int id) {
-    const bool ok = cbeg + t * C + cs < cend;
-    const int cb = cs >> 4, i = cs & 15;
+  // Wave w < CB, lanes 0..15 own the norm and id slots of candidate block w.
+  const bool nslot = NB == 1 && w < CB && skk == 0;
+  auto own_id = [&](const Ids &ids) -> int {
+    int r = ids.v[0];
 #pragma unroll
-    for (int e = 0; e < DPT; ++e) {
-      const int p = p0 + e;
-      const int s = p >> 2, kk = p & 3;
-      tA[(s * CB + cb) * 64 + i + 16 * kk] = (ok && p < d) ? stage[e] : 0.0;
+    for (int cb = 1; cb < CB; ++cb)
+      if (w == cb) r = ids.v[cb];
+    return r;
+  };
+  auto load_x = [&](const Ids &ids) {
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int sb = bidx / CB, cb = bidx % CB;
+      const int p = 4 * sb + skk;
+      stage[e] = a.XA[(size_t)ids.v[cb] * d + (p < d ? p : d - 1)];
     }
-    if (tid % TPC == 0) {
+    if (nslot) stage_n = a.XA2[own_id(ids)];
+  };
+  auto store_x = [&](int buf, int t, const Ids &ids) {
+    double *dst = tA + buf * TILE;
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int sb = bidx / CB, cb = bidx % CB;
+      const int p = 4 * sb + skk;
+      const bool ok = cbeg + t * C + cb * 16 + si < cend;
+      dst[bidx * 64 + lane] = (ok && p < d) ? stage[e] : 0.0;
+    }
+    if (nslot) {
       // An empty slot gets distance +inf and index INT_MAX: it loses to
       // every row entry, including (+inf, -1) placeholders.
-      tn[cs] = ok ? stage_n : __builtin_inf();
-      ti[cs] = ok ? id : INT_MAX;
+      const int slot = w * 16 + si;
+      const bool ok = cbeg + t * C + slot < cend;
+      tn[slot] = ok ? stage_n : __builtin_inf();
+      ti[slot] = ok ? own_id(ids) : INT_MAX;
     }
   };
+  // NB = 2: this lane's tile candidate (lane % C), id two tiles ahead and
+  // norm one tile ahead; empty slots as above.
+  const int cl = lane & (C - 1);
+  auto lane_id = [&](int t) -> int {
+    const int c = min(cbeg + t * C + cl, cend - 1);
+    return a.amap[c];
+  };
+  double cur_tn = __builtin_inf();
+  int cur_ti = INT_MAX;
+  int lid_a = 0, lid_b = 0;  // lane ids of tiles t + 1 and t + 2
+  if (NB == 2 && ntiles > 0) {
+    const int id0 = lane_id(0);
+    const bool ok = cbeg + cl < cend;
+    cur_tn = ok ? a.XA2[id0] : __builtin_inf();
+    cur_ti = ok ? id0 : INT_MAX;
+    lid_a = ntiles > 1 ? lane_id(1) : id0;
+    lid_b = ntiles > 2 ? lane_id(2) : lid_a;
+  }
 
-  int id_n = 0;
+  Ids id_a{}, id_b{};  // staging ids of the tiles after the current one
   if (ntiles > 0) {
-    const int id0 = cand_id(0);
+    const Ids id0 = cand_id(0);
     load_x(id0);
-    id_n = ntiles > 1 ? cand_id(1) : id0;
+    id_a = ntiles > 1 ? cand_id(1) : id0;
     __syncthreads();  // heaps loaded
-    store_x(0, id0);
+    store_x(0, 0, id0);
+    if (NB == 2) {
+      if (ntiles > 1) load_x(id_a);
+      id_b = ntiles > 2 ? cand_id(2) : id_a;
+    }
   }
   __syncthreads();
-  double rd[QB];
+  // The filter compares t = fl(-2 dot + XA2[amap[i]]) with a per-row
+  // threshold instead of the distance fl(t + qn) with the root r.  If
+  // fl(t + qn) <= r then t + qn <= r + 2u|r| (u = 2^-53), so every such pair
+  // has t <= thr for thr = fl(fl(r - qn) + 2^-50 (|r| + |qn|) + 2^-1020),
+  // whose rounding errors the margin covers.  +inf roots give +inf; lanes
+  // without a query get -inf.  Pairs that pass are tested exactly.
+  auto root_thr = [&](int qb) -> double {
+    if (!qv[qb]) return -__builtin_inf();
+    const double r = hd[ql[qb]];
+    const double m = xadd(xmul(0x1p-50, xadd(__builtin_fabs(r), __builtin_fabs(qn[qb]))), 0x1p-1020);
+    return xadd(xadd(r, -qn[qb]), m);
+  };
+  double thr[QB];
 #pragma unroll
-  for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
+  for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
 
-  constexpr int QR = 4;
+  // Per-lane survivor queue: distance and id, with the query block in the
+  // id's top bit (ids are non-negative).
+  static_assert(QB <= 2, "query block fits one bit");
+  constexpr int QR = 6;
   double pq_d[QR];
-  int pq_i[QR], pq_q[QR];
+  unsigned pq_i[QR];
   int pq_n = 0;
 #pragma unroll
   for (int r = 0; r < QR; ++r) {
     pq_d[r] = 0.0;
-    pq_i[r] = 0;
-    pq_q[r] = 0;
+    pq_i[r] = 0u;
   }
-  // Apply the queued survivors and this tile's (pass bits over acc) to the
-  // heaps.  The four lane groups cover the wave's queries once each, so
-  // within a group every lane owns a distinct heap; groups take turns.
-  auto flush = [&](const unsigned *pass, const dbl4 (*acc)[CB]) {
+  // Apply the queued survivors and this tile's (pass bits over acc, ids in
+  // the tile's id slots) to the heaps.  The four lane groups cover the
+  // wave's queries once each, so within a group every lane owns a distinct
+  // heap; groups take turns.
+  auto flush = [&](const unsigned *pass, const dbl4 (*acc)[CB], const idv cid)
+                   __attribute__((always_inline)) {
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       if (grp == g) {
 #pragma unroll
         for (int r = 0; r < QR; ++r) {
           if (r < pq_n) {
-            int slot = ql[0];
-#pragma unroll
-            for (int qb = 1; qb < QB; ++qb)
-              if (pq_q[r] == qb) slot = ql[qb];
-            if (cless(pq_d[r], pq_i[r], hd[slot], hi[slot]))
-              sift_down(hd, hi, Q, slot, 0, k, pq_d[r], pq_i[r]);
+            const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
+            const int id = (int)(pq_i[r] & 0x7fffffffu);
+            if (cless(pq_d[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, pq_d[r], id);
           }
         }
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass[qb];
+          unsigned bits = pass ? pass[qb] : 0u;
           const int slot = ql[qb];
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
             double dist = acc[qb][0][0];
+            int idx = cid[0];
 #pragma unroll
             for (int bb = 1; bb < CB * 4; ++bb)
-              if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
-            const int idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+              if (b == bb) {
+                dist = acc[qb][bb >> 2][bb & 3];
+                idx = cid[bb];
+              }
             if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
           }
         }
@@ -214,95 +305,154 @@ __global__ __launch_bounds__(64 * W) void knn_mfma_f64_kernel(KnnArgs<double> a)
     }
     pq_n = 0;
 #pragma unroll
-    for (int qb = 0; qb < QB; ++qb) rd[qb] = qv[qb] ? hd[ql[qb]] : -__builtin_inf();
+    for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
   };
 
-  for (int t = 0; t < ntiles; ++t) {
-    const bool more = t + 1 < ntiles;
-    const int id_nn = (t + 2 < ntiles) ? cand_id(t + 2) : id_n;
-    if (more) load_x(id_n);
-
-    dbl4 acc[QB][CB];
+  dbl4 acc[QB][CB];
+  // MFMAs of candidate blocks [LO, HI) of the tile at `cur`.
+  auto mfma_blocks = [&](const double *cur, auto lo_c, auto hi_c) __attribute__((always_inline)) {
+    constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int cb = LO; cb < HI; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        const double av = tA[(s * CB + cb) * 64 + lane];
+      for (int cb = LO; cb < HI; ++cb) {
+        const double av = cur[(s * CB + cb) * 64 + lane];
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
-
-    // Hot path: one FP64 compare per pair against the row's current root;
-    // the exact (distance, index) test runs only for the rare survivors.
-    unsigned pass[QB];
-    bool anyp = false;
+  };
+  // Epilogue of candidate blocks [LO, HI): one FMA and one compare per pair,
+  // the compare's wave mask kept in scalar registers; per-lane bits and
+  // exact distances are built only when some pair passes.  Norms and ids
+  // come from LDS (NB = 1) or from the lanes' registers tnr/tir (NB = 2).
+  auto epilogue = [&](auto lo_c, auto hi_c, double tnr, int tir) __attribute__((always_inline)) {
+    constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
+    double ctn[CB * 4];
 #pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-      pass[qb] = 0u;
+    for (int b = LO * 4; b < HI * 4; ++b) {
+      const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
+      ctn[b] = NB == 2 ? __shfl(tnr, c) : tn[c];
+    }
+    unsigned long long msk[QB][CB * 4];
+    unsigned long long anym = 0;
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb)
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = LO; cb < HI; ++cb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int c = cb * 16 + grp + 4 * v;
-          // (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]]; -2 * dot is exact.
-          const double dist = xadd(__builtin_fma(acc[qb][cb][v], -2.0, tn[c]), qn[qb]);
-          acc[qb][cb][v] = dist;
-          if (dist <= rd[qb]) {
-            pass[qb] |= 1u << (cb * 4 + v);
-            anyp = true;
-          }
+          // -2 * dot is exact: t = fl(-2 dot + XA2[amap[i]]).
+          const double tt = __builtin_fma(acc[qb][cb][v], -2.0, ctn[cb * 4 + v]);
+          acc[qb][cb][v] = tt;
+          msk[qb][cb * 4 + v] = __ballot(tt <= thr[qb]);
+          anym |= msk[qb][cb * 4 + v];
         }
-    }
     // Survivors wait in a per-lane queue of QR entries; the heaps (and the
-    // roots used by the filter) are updated only when some lane's queue
-    // would overflow, which keeps the slow path off most tiles.  A stale
-    // root only lets more pairs through the filter; every queued pair is
-    // tested exactly against the live root when it is applied.
-    int npass = 0;
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
-    if (__any(pq_n + npass > QR)) {
-      flush(pass, acc);
-    } else if (__any(anyp)) {
+    // thresholds used by the filter) are updated only when some lane's
+    // queue would overflow, which keeps the slow path off most tiles.  A
+    // stale threshold only lets more pairs through the filter; every queued
+    // pair is tested exactly against the live root when it is applied.
+    if (anym) {
+      unsigned pass[QB];
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
-        unsigned bits = pass[qb];
-        while (bits) {
-          const int b = __builtin_ctz(bits);
-          bits &= bits - 1;
-          double dist = acc[qb][0][0];
+        pass[qb] = 0u;
 #pragma unroll
-          for (int bb = 1; bb < CB * 4; ++bb)
-            if (b == bb) dist = acc[qb][bb >> 2][bb & 3];
-          const int idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+        for (int b = LO * 4; b < HI * 4; ++b)
+          pass[qb] |= (unsigned)((msk[qb][b] >> lane) & 1ull) << b;
 #pragma unroll
-          for (int r = 0; r < QR; ++r)
-            if (pq_n == r) {
-              pq_d[r] = dist;
-              pq_i[r] = idx;
-              pq_q[r] = qb;
-            }
-          ++pq_n;
+        for (int cb = LO; cb < HI; ++cb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[qb][cb][v] = xadd(acc[qb][cb][v], qn[qb]);
+      }
+      idv cid = {};
+#pragma unroll
+      for (int b = LO * 4; b < HI * 4; ++b) {
+        const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
+        cid[b] = NB == 2 ? __shfl(tir, c) : ti[c];
+      }
+      int npass = 0;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
+      if (__any(pq_n + npass > QR)) {
+        flush(pass, acc, cid);
+      } else {
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            double dist = acc[qb][LO][0];
+            int idx = cid[LO * 4];
+#pragma unroll
+            for (int bb = LO * 4 + 1; bb < HI * 4; ++bb)
+              if (b == bb) {
+                dist = acc[qb][bb >> 2][bb & 3];
+                idx = cid[bb];
+              }
+#pragma unroll
+            for (int r = 0; r < QR; ++r)
+              if (pq_n == r) {
+                pq_d[r] = dist;
+                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+              }
+            ++pq_n;
+          }
         }
       }
     }
-    __syncthreads();  // tile consumed
-    if (more) store_x(t + 1, id_n);
-    __syncthreads();  // next tile visible
-    id_n = id_nn;
+  };
+  typedef std::integral_constant<int, 0> I0;
+  typedef std::integral_constant<int, CB> ICB;
+  for (int t = 0; t < ntiles; ++t) {
+    const bool more = t + 1 < ntiles;
+    const double *cur = tA + (NB == 2 ? (t & 1) * TILE : 0);
+    // The amap entry two tiles ahead is requested before the rows of the
+    // next tile, so the row loads never wait behind it.
+    Ids id_c{};
+    int lid_c = 0;
+    double nxt_tn = 0.0;
+    if (NB == 2) {
+      // The staged tile t + 1 goes to the other buffer, whose last readers
+      // (tile t - 1) passed the barrier at the end of the previous step.
+      if (more) store_x((t + 1) & 1, t + 1, id_a);
+      id_c = (t + 3 < ntiles) ? cand_id(t + 3) : id_b;
+      if (t + 2 < ntiles) load_x(id_b);
+      lid_c = (t + 3 < ntiles) ? lane_id(t + 3) : lid_b;
+      nxt_tn = a.XA2[lid_a];
+    } else {
+      id_c = (t + 2 < ntiles) ? cand_id(t + 2) : id_a;
+      if (more) load_x(id_a);
+    }
+
+    mfma_blocks(cur, I0{}, ICB{});
+    epilogue(I0{}, ICB{}, cur_tn, cur_ti);
+
+    if (NB == 2) {
+      __syncthreads();  // tile t consumed, tile t + 1 visible
+      id_a = id_b;
+      id_b = id_c;
+      const bool nxt_ok = more && cbeg + (t + 1) * C + cl < cend;
+      cur_tn = nxt_ok ? nxt_tn : __builtin_inf();
+      cur_ti = nxt_ok ? lid_a : INT_MAX;
+      lid_a = lid_b;
+      lid_b = lid_c;
+    } else {
+      __syncthreads();  // tile consumed
+      if (more) store_x(0, t + 1, id_a);
+      __syncthreads();  // next tile visible
+      id_a = id_c;
+    }
   }
 
-  {
-    const unsigned none[QB] = {};
-    const dbl4 zero[QB][CB] = {};
-    if (__any(pq_n > 0)) flush(none, zero);
-  }
+  if (__any(pq_n > 0)) flush(nullptr, acc, idv{});
   __syncthreads();
   if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
 }
@@ -412,7 +562,7 @@ struct Plan {
   int kind = 0;  // 0 exact, 1 mfma
   int qpb = 64;  // queries per workgroup
   int ct = 0;    // exact: candidates per tile
-  int ks = 0, qb = 0, w = 0;  // mfma
+  int ks = 0, qb = 0, w = 0, nb = 1;  // mfma
   size_t lds = 0;
   int gx = 0, nsplit = 1, cps = 0;
 };
@@ -444,6 +594,69 @@ int merge_qpb(int k, int elem) {
   return 0;
 }
 
+bool verbose() {
+  static const bool v = [] {
+    const char *e = getenv("GSKNN_VERBOSE");
+    return e && *e && *e != '0';
+  }();
+  return v;
+}
+
+// MFMA launch shapes, in order of preference at equal occupancy: more
+// queries per wave (each LDS fragment feeds QB MFMAs), then two tile buffers.
+constexpr int kMfmaCfg[7][3] = {{2, 4, 1}, {2, 4, 2}, {2, 8, 2}, {1, 4, 2}, {1, 4, 1}, {1, 2, 1}, {1, 1, 1}};
+constexpr int kNumMfmaCfg = 7;
+
+// Workgroups per CU that registers allow; 0 for an instantiation that
+// spills to scratch (never chosen).
+template <int KS, int QB, int W, int NB>
+int mfma_reg_blocks() {
+  const void *kern = (const void *)knn_mfma_f64_kernel<KS, QB, mfma_cb(KS), W, NB>;
+  hipFuncAttributes at;
+  if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, 64 * W, 0) != hipSuccess || b <= 0)
+    b = 8 / W;  // two waves per SIMD
+  return b;
+}
+
+template <int KS, int C>
+int mfma_reg_blocks_cfg() {
+  return mfma_reg_blocks<KS, kMfmaCfg[C][0], kMfmaCfg[C][1], kMfmaCfg[C][2]>();
+}
+
+// Workgroups per CU that registers and wave slots allow (LDS aside), per
+// kernel instantiation, queried once.
+int mfma_blocks_per_cu(int ks, int cfg) {
+  static std::atomic<int> cache[5][kNumMfmaCfg];
+  const int ki = ks == 4 ? 0 : ks == 8 ? 1 : ks == 16 ? 2 : ks == 32 ? 3 : 4;
+  int b = cache[ki][cfg].load(std::memory_order_relaxed);  // blocks + 1
+  if (b > 0) return b - 1;
+#define GSKNN_OCC(KS_)                                                     \
+  case KS_:                                                                \
+    switch (cfg) {                                                         \
+      case 0: b = mfma_reg_blocks_cfg<KS_, 0>(); break;                    \
+      case 1: b = mfma_reg_blocks_cfg<KS_, 1>(); break;                    \
+      case 2: b = mfma_reg_blocks_cfg<KS_, 2>(); break;                    \
+      case 3: b = mfma_reg_blocks_cfg<KS_, 3>(); break;                    \
+      case 4: b = mfma_reg_blocks_cfg<KS_, 4>(); break;                    \
+      case 5: b = mfma_reg_blocks_cfg<KS_, 5>(); break;                    \
+      default: b = mfma_reg_blocks_cfg<KS_, 6>(); break;                   \
+    }                                                                      \
+    break;
+  switch (ks) {
+    GSKNN_OCC(4)
+    GSKNN_OCC(8)
+    GSKNN_OCC(16)
+    GSKNN_OCC(32)
+    GSKNN_OCC(64)
+  }
+#undef GSKNN_OCC
+  cache[ki][cfg].store(b + 1, std::memory_order_relaxed);
+  if (verbose()) fprintf(stderr, "gsknn: ks=%d cfg=%d: %d workgroups/CU by registers\n", ks, cfg, b);
+  return b;
+}
+
 Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   Plan p;
   const bool f64 = elem == 8;
@@ -453,20 +666,37 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   if (mfma) {
     p.kind = 1;
     p.ks = ks;
-    const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
-    for (auto &o : opts) {
-      const int Q = o[1] * 16 * o[0];
-      const size_t b = MfmaLayout(ks, mfma_cb(ks), Q, k).bytes;
-      if (b <= kLdsMax) {
-        p.qb = o[0];
-        p.w = o[1];
+    // Score = resident waves per CU (LDS and registers) x QB (MFMAs fed by
+    // one LDS fragment read); ties go to the earlier entry of kMfmaCfg.
+    // GSKNN_MFMA_CFG=<index> pins one shape (tuning).
+    static const int pinned = [] {
+      const char *e = getenv("GSKNN_MFMA_CFG");
+      return e ? atoi(e) : -1;
+    }();
+    int best = 0;
+    for (int c = 0; c < kNumMfmaCfg; ++c) {
+      if (pinned >= 0 && pinned < kNumMfmaCfg && c != pinned) continue;
+      const int qb = kMfmaCfg[c][0], w = kMfmaCfg[c][1], nb = kMfmaCfg[c][2];
+      const int Q = w * 16 * qb;
+      const size_t b = MfmaLayout(ks, mfma_cb(ks), Q, k, nb).bytes;
+      if (b > kLdsMax) continue;
+      const int by_lds = (int)(kLdsMax / b);
+      const int blocks = std::min(by_lds, mfma_blocks_per_cu(ks, c));
+      const int score = blocks * w * qb;
+      if (score > best) {
+        best = score;
+        p.qb = qb;
+        p.w = w;
+        p.nb = nb;
         p.qpb = Q;
         p.lds = b;
-        break;
       }
     }
     if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
     tile_c = 16 * mfma_cb(ks);
+    if (verbose())
+      fprintf(stderr, "gsknn: mfma plan d=%d k=%d: ks=%d qb=%d w=%d nb=%d lds=%zu score=%d\n", d,
+              k, ks, p.qb, p.w, p.nb, p.lds, best);
   }
   if (p.kind == 0) {
     p.ct = exact_ct(d, elem);
@@ -517,17 +747,20 @@ template <int KS>
 hipError_t launch_mfma_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
   const dim3 grid(p.gx, p.nsplit);
   hipError_t e = hipSuccess;
-#define GSKNN_MFMA_CASE(QB_, W_)                                              \
-  if (p.qb == QB_ && p.w == W_) {                                             \
-    auto kern = knn_mfma_f64_kernel<KS, QB_, mfma_cb(KS), W_>;                \
+#define GSKNN_MFMA_CASE(QB_, W_, NB_)                                         \
+  if (p.qb == QB_ && p.w == W_ && p.nb == NB_) {                              \
+    auto kern = knn_mfma_f64_kernel<KS, QB_, mfma_cb(KS), W_, NB_>;           \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
     return hipGetLastError();                                                 \
   }
-  GSKNN_MFMA_CASE(2, 4)
-  GSKNN_MFMA_CASE(1, 4)
-  GSKNN_MFMA_CASE(1, 2)
-  GSKNN_MFMA_CASE(1, 1)
+  GSKNN_MFMA_CASE(2, 8, 2)
+  GSKNN_MFMA_CASE(2, 4, 2)
+  GSKNN_MFMA_CASE(2, 4, 1)
+  GSKNN_MFMA_CASE(1, 4, 2)
+  GSKNN_MFMA_CASE(1, 4, 1)
+  GSKNN_MFMA_CASE(1, 2, 1)
+  GSKNN_MFMA_CASE(1, 1, 1)
 #undef GSKNN_MFMA_CASE
   return hipErrorNotSupported;
 }
