@@ -391,6 +391,20 @@ def main():
     parity = bool(np.array_equal(Is, Irs))
     log(f"spot parity on {rows.size} rows: {parity}")
 
+    # Sorted-output step on the call's rows (heap rows -> ascending order).
+    Ds, Is_ = D.clone(), I.clone()
+    gsknn.rows_ascending_dev(Ds, Is_, stream=stream)
+    ev_s = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    Ds.copy_(D)
+    Is_.copy_(I)
+    ev_s[0].record(stream)
+    gsknn.rows_ascending_dev(Ds, Is_, stream=stream)
+    ev_s[1].record(stream)
+    torch.cuda.synchronize()
+    sort_ms = ev_s[0].elapsed_time(ev_s[1])
+    sort_ok = bool(np.array_equal(Is_.cpu().numpy()[rows], Is))
+    log(f"rows ascending {sort_ms:.3f} ms (matches oracle order: {sort_ok})")
+
     # Host-pointer call through the reference API (dgsknn on host arrays):
     # stages the table rows, maps and heap rows over PCIe and back each call.
     # Reported beside the device rate, never as the value.
@@ -433,6 +447,7 @@ def main():
         "spot_parity": parity,
         "pcie_inclusive": {"ms": host_ms, "pairs_per_s": pairs / (host_ms * 1e-3),
                            "rows_equal_device": host_equal},
+        "rows_ascending": {"ms": sort_ms, "rows": p.n, "k": p.k, "order_ok": sort_ok},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

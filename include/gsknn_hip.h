@@ -106,6 +106,15 @@ int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
                             const int *nitems, double *D, int *I, int ldk,
                             int qpb, int ct, int grid, void *stream);
 
+/* Sorted output: the n heap rows (row j at D + j*ldk, I + j*ldk, k <= 4096)
+ * rewritten in place in ascending (distance, index) order, e.g. after a
+ * gsknn call.  No entry point of the reference library does this; its tree
+ * codes extract each heap to ascending (distance, index) order on the host
+ * (frame/old/dgsrnn_directKQuery.cpp:110-123).  Unfilled (+inf, -1) slots
+ * sort after every finite entry. */
+int gsknn_rows_ascending_ddev(int n, int k, double *D, int *I, int ldk, void *stream);
+int gsknn_rows_ascending_sdev(int n, int k, float *D, int *I, int ldk, void *stream);
+
 /* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
 int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);
 int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *stream);

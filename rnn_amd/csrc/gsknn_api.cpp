@@ -139,6 +139,24 @@ extern "C" int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *
   return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_ssqnorm_dev");
 }
 
+extern "C" int gsknn_rows_ascending_ddev(int n, int k, double *D, int *I, int ldk, void *stream) {
+  if (n < 0 || k < 0 || (n > 0 && k > 0 && (ldk < k || !D || !I)))
+    return set_err(GSKNN_EINVAL, "gsknn_rows_ascending_ddev: n=%d k=%d ldk=%d", n, k, ldk);
+  if (k > kRowsAscendingMaxK)
+    return set_err(GSKNN_ELIMIT, "gsknn_rows_ascending_ddev: k=%d > %d", k, kRowsAscendingMaxK);
+  hipError_t e = launch_rows_ascending_f64(n, k, D, I, ldk, (hipStream_t)stream);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_rows_ascending_ddev");
+}
+
+extern "C" int gsknn_rows_ascending_sdev(int n, int k, float *D, int *I, int ldk, void *stream) {
+  if (n < 0 || k < 0 || (n > 0 && k > 0 && (ldk < k || !D || !I)))
+    return set_err(GSKNN_EINVAL, "gsknn_rows_ascending_sdev: n=%d k=%d ldk=%d", n, k, ldk);
+  if (k > kRowsAscendingMaxK)
+    return set_err(GSKNN_ELIMIT, "gsknn_rows_ascending_sdev: k=%d > %d", k, kRowsAscendingMaxK);
+  hipError_t e = launch_rows_ascending_f32(n, k, D, I, ldk, (hipStream_t)stream);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_rows_ascending_sdev");
+}
+
 extern "C" size_t gsknn_bf16_prep_bytes(int N, int d) {
   return (N > 0 && d > 0) ? bf16_prep_bytes(N, d) : 0;
 }

@@ -53,6 +53,11 @@ hipError_t launch_hdr_to_rows(int n, int k, int ldk, const double *Dh, const int
 hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int *It,
                               double *Dh, int *Ih, hipStream_t s);
 
+// Rows to ascending (distance, index) order in place (k <= 4096).
+constexpr int kRowsAscendingMaxK = 4096;
+hipError_t launch_rows_ascending_f64(int n, int k, double *D, int *I, int ldk, hipStream_t s);
+hipError_t launch_rows_ascending_f32(int n, int k, float *D, int *I, int ldk, hipStream_t s);
+
 // FP32 MFMA candidate pass + re-rank in the oracle's FP32 arithmetic
 // (gsknn_f32.hip).
 size_t f32_work_bytes(int m, int n, int d, int k);

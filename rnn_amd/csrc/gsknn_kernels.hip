@@ -491,6 +491,51 @@ __global__ __launch_bounds__(QPB) void knn_merge_kernel(KnnArgs<T> a) {
   heap_store(b, hd, hi, QPB, q, j, 0);
 }
 
+// Rows to ascending (distance, index) order, in place: each 64-lane
+// workgroup sorts R rows of P = next power of two >= k entries with a
+// bitonic network in LDS (padding (+inf, INT_MAX) sorts last).
+template <typename T>
+__global__ __launch_bounds__(64) void rows_ascending_kernel(int n, int k, T *D, int *I, int ldk,
+                                                            int P, int R) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *sd = (T *)smem;
+  int *si = (int *)(smem + al16(sizeof(T) * (size_t)P * R));
+  const int r0 = blockIdx.x * R;
+  for (int t = threadIdx.x; t < P * R; t += 64) {
+    const int r = r0 + t / P, e = t % P;
+    const bool v = r < n && e < k;
+    sd[t] = v ? D[(size_t)r * ldk + e] : (T)__builtin_inf();
+    si[t] = v ? I[(size_t)r * ldk + e] : INT_MAX;
+  }
+  __syncthreads();
+  const int half = P / 2;
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < half * R; t += 64) {
+        const int base = (t / half) * P, q = t % half;
+        const int lo = 2 * stride * (q / stride) + q % stride;
+        const int hi = lo + stride;
+        const T a = sd[base + lo], b = sd[base + hi];
+        const int ia = si[base + lo], ib = si[base + hi];
+        const bool up = (lo & size) == 0;
+        if (up ? cless(b, ib, a, ia) : cless(a, ia, b, ib)) {
+          sd[base + lo] = b;
+          sd[base + hi] = a;
+          si[base + lo] = ib;
+          si[base + hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = threadIdx.x; t < P * R; t += 64) {
+    const int r = r0 + t / P, e = t % P;
+    if (r < n && e < k) {
+      D[(size_t)r * ldk + e] = sd[t];
+      I[(size_t)r * ldk + e] = si[t];
+    }
+  }
+}
+
 template <typename T>
 __global__ void sqnorm_kernel(int N, int d, const T *X, T *X2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -910,6 +955,30 @@ hipError_t launch_rows_to_hdr(int n, int k, int ldk, const double *Dt, const int
   hipLaunchKernelGGL(rows_to_hdr_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, k, ldk, Dt, It,
                      Dh, Ih);
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rows_ascending_t(int n, int k, T *D, int *I, int ldk, hipStream_t s) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  if (k > kRowsAscendingMaxK || ldk < k) return hipErrorInvalidValue;
+  int P = 1;
+  while (P < k) P <<= 1;
+  const int R = P >= 64 ? 1 : 64 / P;  // rows per workgroup
+  const size_t lds = al16(sizeof(T) * (size_t)P * R) + sizeof(int) * (size_t)P * R;
+  hipError_t e = hipFuncSetAttribute((const void *)rows_ascending_kernel<T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rows_ascending_kernel<T>, dim3((n + R - 1) / R), dim3(64), lds, s, n, k, D, I,
+                     ldk, P, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_ascending_f64(int n, int k, double *D, int *I, int ldk, hipStream_t s) {
+  return launch_rows_ascending_t<double>(n, k, D, I, ldk, s);
+}
+
+hipError_t launch_rows_ascending_f32(int n, int k, float *D, int *I, int ldk, hipStream_t s) {
+  return launch_rows_ascending_t<float>(n, k, D, I, ldk, s);
 }
 
 hipError_t launch_sqnorm_f64(int N, int d, const double *X, double *X2, hipStream_t s) {

@@ -76,6 +76,12 @@ __global__ void prep_zero_kernel(float *mx) {
   if (threadIdx.x < 4) mx[threadIdx.x] = 0.0f;
 }
 
+// Counters are cleared by a kernel, not hipMemsetAsync, so that a launch
+// sequence captured into a HIP graph replays correctly.
+__global__ void zero_int_kernel(int *p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 __global__ void prep_bf16_kernel(int N, int d, const float *X, unsigned char *prep) {
   const PrepLayout L(N, d);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -713,7 +719,8 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   a.cnt = (int *)((char *)work + o_cnt);
   a.U = (float *)((char *)work + o_u);
   a.nover = (int *)((char *)work + o_over);
-  hipError_t e = hipMemsetAsync(a.nover, 0, sizeof(int), s);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, a.nover);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   switch (p.ks) {
     case 1: e = launch_filter_ks<1>(p, a, s); break;

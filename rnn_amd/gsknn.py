@@ -83,6 +83,8 @@ def _declare(L):
     L.gsknn_bf16_knn_dev.argtypes = [i32] * 5 + [vp] * 3 + [i32] + [vp] * 5 + [i32, vp, vp, sz, vp, vp]
     L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
+    L.gsknn_rows_ascending_ddev.argtypes = [i32, i32, vp, vp, i32, vp]
+    L.gsknn_rows_ascending_sdev.argtypes = [i32, i32, vp, vp, i32, vp]
     for name in ("dgsknn_var1", "sgsknn_var1"):
         getattr(L, name).argtypes = [i32] * 4 + [vp] * 8
         getattr(L, name).restype = None
@@ -244,6 +246,18 @@ def work_bytes_s(m, n, d, k, mode=MODE_AUTO) -> int:
 def sqnorm_dev(X, X2, stream=None):
     fn = lib().gsknn_dsqnorm_dev if X.element_size() == 8 else lib().gsknn_ssqnorm_dev
     rc = fn(X.shape[0], X.shape[1], _dp(X), _dp(X2), _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+def rows_ascending_dev(D, I, stream=None):
+    """Rewrite heap rows (n, k) of device tensors D/I in place in ascending
+    (distance, index) order -- the sorted-output step tree codes apply to
+    gsknn heaps (frame/old/dgsrnn_directKQuery.cpp:110-123)."""
+    assert D.dim() == 2 and D.shape == I.shape and D.stride(1) == 1 and I.stride(1) == 1
+    assert D.stride(0) == I.stride(0)
+    fn = lib().gsknn_rows_ascending_ddev if D.element_size() == 8 else lib().gsknn_rows_ascending_sdev
+    rc = fn(D.shape[0], D.shape[1], _dp(D), _dp(I), D.stride(0), _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
 

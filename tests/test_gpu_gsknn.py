@@ -410,3 +410,55 @@ def test_bf16_and_fp32_few_queries_many_candidates():
     Dr, Ir = Df.copy(), If.copy()
     oracle.gsknn(X, X2f, amap, X, X2f, bmap, Dr, Ir, canonical=True)
     assert_rows(tD.cpu().numpy(), tI.cpu().numpy(), Dr, Ir)
+
+
+@pytest.mark.parametrize("k,ldk,dtype", [(1, 1, np.float64), (7, 9, np.float32), (32, 32, np.float64),
+                                         (64, 70, np.float32), (600, 600, np.float64),
+                                         (4096, 4096, np.float32)])
+def test_rows_ascending_matches_oracle_sort(k, ldk, dtype):
+    # Sorted output on the device == the oracle's (distance, index) sort, on
+    # rows with duplicates, exact ties broken by index, and (+inf, -1) slots.
+    import torch
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(k * 7 + ldk))
+    n = 257
+    print(f"[gpu] rows_ascending k={k} ldk={ldk} {np.dtype(dtype).name}", flush=True)
+    D = np.round(g.random((n, ldk)) * 50).astype(dtype)       # many exact ties
+    I = g.integers(0, 1000, size=(n, ldk)).astype(np.int32)
+    D[::5, : k // 3] = np.inf                                  # unfilled slots
+    I[::5, : k // 3] = -1
+    if ldk > 1:
+        D[1::7, 0] = D[1::7, 1]                                # duplicate entries
+        I[1::7, 0] = I[1::7, 1]
+    tD, tI = torch.from_numpy(D.copy()).to(dev), torch.from_numpy(I.copy()).to(dev)
+    gsknn.rows_ascending_dev(tD[:, :k], tI[:, :k])
+    torch.cuda.synchronize()
+    Dg, Ig = tD.cpu().numpy(), tI.cpu().numpy()
+    Dr, Ir = oracle.sort_rows(D[:, :k].copy(), I[:, :k].copy())
+    assert np.array_equal(Dg[:, :k].view(np.uint8), Dr.view(np.uint8))
+    np.testing.assert_array_equal(Ig[:, :k], Ir)
+    np.testing.assert_array_equal(Dg[:, k:], D[:, k:])        # padding columns untouched
+    np.testing.assert_array_equal(Ig[:, k:], I[:, k:])
+
+
+def test_rows_ascending_after_knn_and_example():
+    # gsknn rows -> ascending order equals the oracle's sorted rows; the
+    # Python 3 port of the reference example runs and checks itself.
+    import torch
+    from rnn_amd import example
+    dev = torch.device("cuda")
+    p = ds.config1(scale=4)
+    D, I = ds.empty_heap(p.n, p.k)
+    tX, tX2 = torch.from_numpy(p.XA).to(dev), torch.from_numpy(p.XA2).to(dev)
+    tD, tI = torch.from_numpy(D.copy()).to(dev), torch.from_numpy(I.copy()).to(dev)
+    gsknn.dknn_dev(tX, tX2, torch.from_numpy(p.amap).to(dev), tX, tX2,
+                   torch.from_numpy(p.bmap).to(dev), tD, tI)
+    gsknn.rows_ascending_dev(tD, tI)
+    torch.cuda.synchronize()
+    Dr, Ir = D.copy(), I.copy()
+    oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, p.bmap, Dr, Ir, canonical=True)
+    Dr, Ir = oracle.sort_rows(Dr, Ir)
+    np.testing.assert_array_equal(tI.cpu().numpy(), Ir)
+    assert np.array_equal(tD.cpu().numpy().view(np.uint8), Dr.view(np.uint8))
+    assert example.main([]) == 0
+    assert example.main(["--double"]) == 0
