@@ -264,10 +264,29 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     pq_d[r] = 0.0;
     pq_i[r] = 0u;
   }
+  // Result value b (= cb * 4 + v) of query block qb: the exact distance
+  // fl(t + |b|^2) from acc's t = fl(-2 dot + |a|^2), and the candidate id
+  // (from the tile's id slots in LDS, or from cid when NB = 2).
+  auto survivor = [&](int qb, int b, const dbl4 (*acc)[CB], const idv cid, double &dist, int &idx)
+                      __attribute__((always_inline)) {
+    double t = acc[qb][0][0];
+#pragma unroll
+    for (int bb = 1; bb < CB * 4; ++bb)
+      if (b == bb) t = acc[qb][bb >> 2][bb & 3];
+    dist = xadd(t, qn[qb]);
+    if (NB == 2) {
+      idx = cid[0];
+#pragma unroll
+      for (int bb = 1; bb < CB * 4; ++bb)
+        if (b == bb) idx = cid[bb];
+    } else {
+      idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+    }
+  };
   // Apply the queued survivors and this tile's (pass bits over acc, ids in
-  // the tile's id slots) to the heaps.  The four lane groups cover the
-  // wave's queries once each, so within a group every lane owns a distinct
-  // heap; groups take turns.
+  // the tile's id slots or cid) to the heaps.  The four lane groups cover
+  // the wave's queries once each, so within a group every lane owns a
+  // distinct heap; groups take turns.
   auto flush = [&](const unsigned *pass, const dbl4 (*acc)[CB], const idv cid)
                    __attribute__((always_inline)) {
 #pragma unroll 1
@@ -288,14 +307,9 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
-            double dist = acc[qb][0][0];
-            int idx = cid[0];
-#pragma unroll
-            for (int bb = 1; bb < CB * 4; ++bb)
-              if (b == bb) {
-                dist = acc[qb][bb >> 2][bb & 3];
-                idx = cid[bb];
-              }
+            double dist;
+            int idx;
+            survivor(qb, b, acc, cid, dist, idx);
             if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
           }
         }
@@ -362,20 +376,21 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       unsigned pass[QB];
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
+        unsigned long long aq = 0ull;  // scalar: any pass in this block
+#pragma unroll
+        for (int b = LO * 4; b < HI * 4; ++b) aq |= msk[qb][b];
         pass[qb] = 0u;
+        if (aq != 0ull) {
 #pragma unroll
-        for (int b = LO * 4; b < HI * 4; ++b)
-          pass[qb] |= (unsigned)((msk[qb][b] >> lane) & 1ull) << b;
-#pragma unroll
-        for (int cb = LO; cb < HI; ++cb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[qb][cb][v] = xadd(acc[qb][cb][v], qn[qb]);
+          for (int b = LO * 4; b < HI * 4; ++b)
+            pass[qb] |= (unsigned)((msk[qb][b] >> lane) & 1ull) << b;
+        }
       }
       idv cid = {};
+      if (NB == 2) {
 #pragma unroll
-      for (int b = LO * 4; b < HI * 4; ++b) {
-        const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
-        cid[b] = NB == 2 ? __shfl(tir, c) : ti[c];
+        for (int b = LO * 4; b < HI * 4; ++b)
+          cid[b] = __shfl(tir, (b >> 2) * 16 + grp + 4 * (b & 3));
       }
       int npass = 0;
 #pragma unroll
@@ -389,14 +404,9 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
-            double dist = acc[qb][LO][0];
-            int idx = cid[LO * 4];
-#pragma unroll
-            for (int bb = LO * 4 + 1; bb < HI * 4; ++bb)
-              if (b == bb) {
-                dist = acc[qb][bb >> 2][bb & 3];
-                idx = cid[bb];
-              }
+            double dist;
+            int idx;
+            survivor(qb, b, acc, cid, dist, idx);
 #pragma unroll
             for (int r = 0; r < QR; ++r)
               if (pq_n == r) {
