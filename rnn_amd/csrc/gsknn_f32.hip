@@ -144,14 +144,14 @@ struct F32Layout {
 // one LDS buffer in MFMA fragment order (lane l of k-step s, block cb holds
 // candidate cb*16 + l%16, dim 4s + l/16), query fragments in registers, a
 // one-compare hot filter and a per-lane deferred queue for the survivors.
+// Two waves per SIMD when the query fragments (QB * KS registers) leave
+// room for the rest.
+constexpr int f32_waves_per_eu(int KS, int QB) { return QB * KS <= 128 ? 2 : 1; }
+
 template <int KS, int QB, int CB, int W>
-__global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
-  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, NT = 64 * W;
-  constexpr int DP = 4 * KS;
-  constexpr int CH = KS;  // 16-byte chunks (4 dims) per candidate row
-  static_assert(NT % C == 0, "threads per candidate");
-  constexpr int TPC = NT / C;
-  constexpr int CPT = (CH + TPC - 1) / TPC;
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filter_kernel(F32Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
   constexpr int QR = 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k, d = a.d;
@@ -220,53 +220,78 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
   }
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
-  const int cs = tid / TPC;
-  const int ch0 = (tid % TPC) * CPT;
-  float4 stage[CPT];
+  // Staging: the tile is BLK fragment blocks of 64 floats (k-step s,
+  // candidate block cb); wave w stages blocks w * BPW .. w * BPW + BPW - 1 and
+  // lane l writes element l of each, i.e. dim 4 s + l / 16 of candidate
+  // cb * 16 + l % 16: every LDS store is 256 contiguous bytes (no bank
+  // conflicts) and a load instruction reads 16 contiguous bytes of each of
+  // 16 candidate rows.  Wave w < CB, lanes 0..15 own the norm slots of
+  // candidate block w.
+  constexpr int BLK = KS * CB;
+  static_assert(BLK % W == 0, "fragment blocks per wave");
+  constexpr int BPW = BLK / W;
+  const int si = lane & 15, skk = lane >> 4;
+  const bool nslot = w < CB && skk == 0;
+  float stage[BPW];
   float st_n = 0.f, st_x = 0.f;
-  auto cand_pos = [&](int t) { return min(cbeg + t * C + cs, cend - 1); };
-  auto load_x = [&](int pos, int id) {
-    const float *row = a.XA + (size_t)id * d;
+  // Ids of this thread's candidates: of the tile being staged (sid) and of
+  // the tile after it (nid, requested a tile ahead so that no row load
+  // waits on its amap entry).
+  typedef int idvec __attribute__((ext_vector_type(CB)));  // keeps selections in registers
+  idvec sid, nid;
+  auto cand_pos = [&](int t, int cb) { return min(cbeg + t * C + cb * 16 + si, cend - 1); };
+  // Id of the candidate staged as element e (block w * BPW + e); constant
+  // register index when BPW is a multiple of CB.
+  auto stage_id = [&](int e) __attribute__((always_inline)) -> int {
+    if (BPW % CB == 0) return sid[e % CB];
+    const int cb = (w * BPW + e) % CB;
+    int r = sid[0];
 #pragma unroll
-    for (int e = 0; e < CPT; ++e) {
-      const int ch = ch0 + e;
-      float4 v;
-      const int p = 4 * ch;
-      v.x = row[min(p, d - 1)];
-      v.y = row[min(p + 1, d - 1)];
-      v.z = row[min(p + 2, d - 1)];
-      v.w = row[min(p + 3, d - 1)];
-      stage[e] = v;
+    for (int c = 1; c < CB; ++c)
+      if (cb == c) r = sid[c];
+    return r;
+  };
+  auto load_ids = [&](int t, idvec &ids) __attribute__((always_inline)) {
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) ids[cb] = a.amap[cand_pos(t, cb)];
+  };
+  auto load_x = [&](int t) {
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int p = 4 * (bidx / CB) + skk;
+      stage[e] = a.XA[(size_t)stage_id(e) * d + (p < d ? p : d - 1)];
     }
-    st_n = a.XA2[id];
-    st_x = a.xnA[pos];
+    if (nslot) {
+      int id = sid[0];
+#pragma unroll
+      for (int cb = 1; cb < CB; ++cb)
+        if (w == cb) id = sid[cb];
+      st_n = a.XA2[id];
+      st_x = a.xnA[cand_pos(t, w)];
+    }
   };
   auto store_x = [&](int t) {
-    const bool ok = cbeg + t * C + cs < cend;
-    const int cb = cs >> 4, i = cs & 15;
 #pragma unroll
-    for (int e = 0; e < CPT; ++e) {
-      const int ch = ch0 + e;
-      if (CH % TPC == 0 || ch < CH) {
-        float *A = tA + (size_t)(ch * CB + cb) * 64 + i;
-        const int p = 4 * ch;
-        A[0] = (ok && p < d) ? stage[e].x : 0.0f;
-        A[16] = (ok && p + 1 < d) ? stage[e].y : 0.0f;
-        A[32] = (ok && p + 2 < d) ? stage[e].z : 0.0f;
-        A[48] = (ok && p + 3 < d) ? stage[e].w : 0.0f;
-      }
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int cb = bidx % CB;
+      const int p = 4 * (bidx / CB) + skk;
+      const bool ok = cbeg + t * C + cb * 16 + si < cend;
+      tA[(size_t)bidx * 64 + lane] = (ok && p < d) ? stage[e] : 0.0f;
     }
-    if (tid % TPC == 0) {
-      tnc[cs] = ok ? st_n : __builtin_inff();
-      txn[cs] = ok ? st_x : 0.0f;
+    if (nslot) {
+      const int slot = w * 16 + si;
+      const bool ok = cbeg + t * C + slot < cend;
+      tnc[slot] = ok ? st_n : __builtin_inff();
+      txn[slot] = ok ? st_x : 0.0f;
     }
   };
 
-  int pos_n = 0, id_n = 0;
   if (ntiles > 0) {
-    load_x(cand_pos(0), a.amap[cand_pos(0)]);
-    pos_n = cand_pos(ntiles > 1 ? 1 : 0);
-    id_n = a.amap[pos_n];
+    load_ids(0, sid);
+    load_ids(ntiles > 1 ? 1 : 0, nid);
+    load_x(0);
     __syncthreads();
     store_x(0);
   }
@@ -320,9 +345,9 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
 
   for (int t = 0; t < ntiles; ++t) {
     const bool more = t + 1 < ntiles;
-    const int pos_nn = cand_pos(t + 2 < ntiles ? t + 2 : t + 1);
-    const int id_nn = a.amap[pos_nn];
-    if (more) load_x(pos_n, id_n);
+    sid = nid;
+    if (t + 2 < ntiles) load_ids(t + 2, nid);
+    if (more) load_x(t + 1);
     f32x4 acc[QB][CB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
@@ -338,11 +363,12 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
           acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
-    unsigned pass[QB];
-    int npass = 0;
+    // Hot path: one FMA and one compare per pair, the compare's wave mask
+    // kept in scalar registers; per-lane bits only when some pair passes.
+    unsigned long long msk[QB][CB * 4];
+    unsigned long long anym = 0ull;
 #pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-      pass[qb] = 0u;
+    for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
@@ -350,11 +376,19 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
           const int c = cb * 16 + 4 * grp + v;
           const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
           acc[qb][cb][v] = tt;
-          if (tt <= thr[qb]) pass[qb] |= 1u << (cb * 4 + v);
+          msk[qb][cb * 4 + v] = __ballot(tt <= thr[qb]);
+          anym |= msk[qb][cb * 4 + v];
         }
-      npass += __builtin_popcount(pass[qb]);
-    }
-    if (__any(npass > 0)) {
+    if (anym) {
+      unsigned pass[QB];
+      int npass = 0;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        pass[qb] = 0u;
+#pragma unroll
+        for (int b = 0; b < CB * 4; ++b) pass[qb] |= (unsigned)((msk[qb][b] >> lane) & 1ull) << b;
+        npass += __builtin_popcount(pass[qb]);
+      }
       if (__any(pq_n + npass > QR)) flush();
       const bool direct = __any(pq_n + npass > QR);
 #pragma unroll 1
@@ -400,8 +434,6 @@ __global__ __launch_bounds__(64 * W) void knn_f32_filter_kernel(F32Args a) {
     __syncthreads();
     if (more) store_x(t + 1);
     __syncthreads();
-    pos_n = pos_nn;
-    id_n = id_nn;
   }
   if (__any(pq_n > 0)) flush();
   __syncthreads();
@@ -542,8 +574,13 @@ F32Plan plan_f32(int d, int k) {
   }
   if (p.ks == 0) return p;
   p.cb = p.ks >= 32 ? 1 : 2;
-  const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
+  // At d > 128 four waves of QB = 2 exceed 256 registers each (the query
+  // fragments alone take 128), so 8-wave workgroups, which stage half as
+  // much per thread, come first there (measured at d = 256: 114 ms against
+  // 166 ms for 4 waves at one wave per SIMD).
+  const int opts[5][2] = {{2, 8}, {2, 4}, {1, 4}, {1, 2}, {1, 1}};
   for (auto &o : opts) {
+    if (o[1] == 8 && p.ks < 64) continue;
     const int Q = o[1] * 16 * o[0];
     const size_t b = F32Layout(p.ks, p.cb, Q, k).bytes;
     if (b <= kLdsF) {
@@ -588,6 +625,7 @@ hipError_t launch_f32_ks(const F32Plan &p, const F32Args &a, hipStream_t s) {
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);                         \
     return hipGetLastError();                                                           \
   }
+  GSKNN_F32_CASE(2, 8)
   GSKNN_F32_CASE(2, 4)
   GSKNN_F32_CASE(1, 4)
   GSKNN_F32_CASE(1, 2)

@@ -180,26 +180,27 @@ __device__ void sift_vals(float *h, int QS, int q, int k, float x) {
 }
 
 // Candidate tiles go global -> registers -> LDS (one buffer, two barriers per
-// tile): thread t stages the 16-byte chunks [ch0, ch0 + CPT) of candidate slot
-// t / TPC; the amap entry of the next tile is read one tile ahead.  Lanes hold
-// their query fragments in registers for the whole launch.
+// tile) in whole fragment blocks per wave; the amap entries of a tile are read
+// one tile ahead.  Lanes hold their query fragments in registers for the
+// whole launch.
 //
-// Hot path per pair: tt = fma(-2, dot~, nc) and one compare against the
-// query's threshold thr = U - nq + delta_max (delta_max bounds delta over all
-// candidates).  Survivors get their exact lower/upper bounds and wait in a
+// Hot path per pair: tt = fma(-2, dot~, nc) and a running minimum; a query
+// block whose minimum reaches the threshold thr = U - nq + delta_max
+// (delta_max bounds delta over all candidates) compares its pairs.  Survivors get their exact lower/upper bounds and wait in a
 // per-lane queue of QR entries; the queue is applied (list append, upper-bound
 // heap update) only when some lane's queue would overflow.  A stale U only
 // admits more candidates; every queued candidate is tested against the live U
 // when applied.
+// Two waves per SIMD when the query fragments (4 * QB * KS registers) leave
+// room for the rest.
+constexpr int bf16_waves_per_eu(int KS, int QB) { return QB * KS <= 16 ? 2 : 1; }
+
 template <int KS, int QB, int CB, int W>
-__global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
-  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB, NT = 64 * W;
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_filter_kernel(Bf16Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
   constexpr int DP = 32 * KS;
   constexpr int TILE = KS * CB * 64 * 8;  // bf16 elements per tile
-  constexpr int CH = DP / 8;              // 16-byte chunks per candidate row
-  static_assert(NT % C == 0, "threads per candidate");
-  constexpr int TPC = NT / C;
-  constexpr int CPT = (CH + TPC - 1) / TPC;
   constexpr int QR = 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k;
@@ -281,46 +282,75 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
   }
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
-  const int cs = tid / TPC;
-  const int ch0 = (tid % TPC) * CPT;
-  uint4 stage[CPT];
+  // Staging: the tile is BLK fragment blocks of 64 x 8 bf16 (k-step s,
+  // candidate block cb); wave w stages blocks w * BPW .. w * BPW + BPW - 1 and
+  // lane l writes 16-byte element l of each (dims 32 s + 8 (l / 16) .. + 8 of
+  // candidate cb * 16 + l % 16): every LDS store is 1 KB contiguous (no bank
+  // conflicts).  Wave w < CB, lanes 0..15 own the norm slots of block w.
+  constexpr int BLK = KS * CB;
+  static_assert(BLK % W == 0, "fragment blocks per wave");
+  constexpr int BPW = BLK / W;
+  const int si = lane & 15, skk = lane >> 4;
+  const bool nslot = w < CB && skk == 0;
+  uint4 stage[BPW];
   float st_n = 0.f, st_x = 0.f, st_e = 0.f;
-  auto cand_pos = [&](int t) { return min(cbeg + t * C + cs, cend - 1); };
-  auto load_x = [&](int id) {
-    const uint4 *row = (const uint4 *)(xbA + (size_t)id * DP);
+  // Ids of this thread's candidates: of the tile being staged (sid) and of
+  // the tile after it (nid, requested a tile ahead).
+  typedef int idvec __attribute__((ext_vector_type(CB)));  // keeps selections in registers
+  idvec sid, nid;
+  auto cand_pos = [&](int t, int cb) { return min(cbeg + t * C + cb * 16 + si, cend - 1); };
+  // Id of the candidate staged as element e (block w * BPW + e); constant
+  // register index when BPW is a multiple of CB.
+  auto stage_id = [&](int e) __attribute__((always_inline)) -> int {
+    if (BPW % CB == 0) return sid[e % CB];
+    const int cb = (w * BPW + e) % CB;
+    int r = sid[0];
 #pragma unroll
-    for (int e = 0; e < CPT; ++e) {
-      const int ch = ch0 + e;
-      stage[e] = row[(CH % TPC == 0 || ch < CH) ? ch : CH - 1];
+    for (int c = 1; c < CB; ++c)
+      if (cb == c) r = sid[c];
+    return r;
+  };
+  auto load_ids = [&](int t, idvec &ids) __attribute__((always_inline)) {
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) ids[cb] = a.amap[cand_pos(t, cb)];
+  };
+  auto load_x = [&]() {
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int ch = 4 * (bidx / CB) + skk;  // 16-byte chunk of the padded row
+      stage[e] = *(const uint4 *)(xbA + (size_t)stage_id(e) * DP + 8 * ch);
     }
-    st_n = (float)n64A[id];
-    st_x = xnA[id];
-    st_e = enA[id];
+    if (nslot) {
+      int id = sid[0];
+#pragma unroll
+      for (int cb = 1; cb < CB; ++cb)
+        if (w == cb) id = sid[cb];
+      st_n = (float)n64A[id];
+      st_x = xnA[id];
+      st_e = enA[id];
+    }
   };
   auto store_x = [&](int t) {
-    const bool ok = cbeg + t * C + cs < cend;
-    const int cb = cs >> 4, i = cs & 15;
 #pragma unroll
-    for (int e = 0; e < CPT; ++e) {
-      const int ch = ch0 + e;
-      if (CH % TPC == 0 || ch < CH) {
-        const int s = ch >> 2, gg = ch & 3;
-        *(uint4 *)(tA + ((size_t)(s * CB + cb) * 64 + i + 16 * gg) * 8) =
-            ok ? stage[e] : make_uint4(0, 0, 0, 0);
-      }
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const bool ok = cbeg + t * C + (bidx % CB) * 16 + si < cend;
+      *(uint4 *)(tA + ((size_t)bidx * 64 + lane) * 8) = ok ? stage[e] : make_uint4(0, 0, 0, 0);
     }
-    if (tid % TPC == 0) {
-      tnc[cs] = ok ? st_n : __builtin_inff();
-      txn[cs] = ok ? st_x : 0.0f;
-      ten[cs] = ok ? st_e : 0.0f;
+    if (nslot) {
+      const int slot = w * 16 + si;
+      const bool ok = cbeg + t * C + slot < cend;
+      tnc[slot] = ok ? st_n : __builtin_inff();
+      txn[slot] = ok ? st_x : 0.0f;
+      ten[slot] = ok ? st_e : 0.0f;
     }
   };
 
-  int id_n = 0;
   if (ntiles > 0) {
-    const int id0 = a.amap[cand_pos(0)];
-    load_x(id0);
-    id_n = a.amap[cand_pos(ntiles > 1 ? 1 : 0)];
+    load_ids(0, sid);
+    load_ids(ntiles > 1 ? 1 : 0, nid);
+    load_x();
     __syncthreads();  // heaps loaded
     store_x(0);
   }
@@ -375,8 +405,9 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
 
   for (int t = 0; t < ntiles; ++t) {
     const bool more = t + 1 < ntiles;
-    const int id_nn = (t + 2 < ntiles) ? a.amap[cand_pos(t + 2)] : id_n;
-    if (more) load_x(id_n);
+    sid = nid;
+    if (t + 2 < ntiles) load_ids(t + 2, nid);
+    if (more) load_x();
     f32x4 acc[QB][CB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
@@ -392,11 +423,12 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
           acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
-    unsigned pass[QB];
-    int npass = 0;
+    // Hot path: one FMA and one min per pair; a query block's pass bits are
+    // built only when its smallest t reaches the threshold.
+    float tmin[QB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
-      pass[qb] = 0u;
+      tmin[qb] = __builtin_inff();
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
@@ -404,11 +436,25 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
           const int c = cb * 16 + out_row(lane, v);
           const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
           acc[qb][cb][v] = tt;
-          if (tt <= thr[qb]) pass[qb] |= 1u << (cb * 4 + v);
+          tmin[qb] = fminf(tmin[qb], tt);
         }
-      npass += __builtin_popcount(pass[qb]);
     }
-    if (__any(npass > 0)) {
+    bool hit = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) hit |= tmin[qb] <= thr[qb];
+    if (__any(hit)) {
+      unsigned pass[QB];
+      int npass = 0;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        pass[qb] = 0u;
+        if (tmin[qb] <= thr[qb]) {
+#pragma unroll
+          for (int b = 0; b < CB * 4; ++b)
+            pass[qb] |= (acc[qb][b >> 2][b & 3] <= thr[qb] ? 1u : 0u) << b;
+        }
+        npass += __builtin_popcount(pass[qb]);
+      }
       if (__any(pq_n + npass > QR)) flush();
       // After a flush the queues are empty; if some lane still cannot queue
       // all of its survivors, every lane applies this tile's survivors now.
@@ -474,7 +520,6 @@ __global__ __launch_bounds__(64 * W) void knn_bf16_filter_kernel(Bf16Args a) {
     __syncthreads();  // tile consumed
     if (more) store_x(t + 1);
     __syncthreads();  // next tile visible
-    id_n = id_nn;
   }
   if (__any(pq_n > 0)) flush();
   __syncthreads();
