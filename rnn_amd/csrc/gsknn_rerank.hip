@@ -29,9 +29,10 @@
 // {upper bounds of all candidates} U {resident heap distances}; U bounds the
 // final k-th distance from above.  Every candidate whose lower bound is <= U
 // at the time it is seen is appended to the query's list (position, lower).
-// Pass 2 (knn_rerank_kernel): one wave per query drops list entries with
-// lower > final U, computes sum_p (a_p - b_p)^2 in FP64 for the rest, and
-// merges them into the resident row under the canonical order.  A query
+// Pass 2 (knn_rerank_kernel): one wave per query reads its list 64 entries
+// at a time, drops entries with lower > final U, computes sum_p (a_p - b_p)^2
+// in FP64 for the rest (four at a time, 16 lanes each, fixed summation order)
+// and merges them into the resident row under the canonical order.  A query
 // whose list overflowed is re-scanned against every candidate in FP64.
 #include <limits.h>
 
@@ -159,6 +160,7 @@ struct FilterLayout {
   }
 };
 
+
 __device__ void sift_vals(float *h, int QS, int q, int k, float x) {
   int s = 0;
   while (true) {
@@ -178,7 +180,6 @@ __device__ void sift_vals(float *h, int QS, int q, int k, float x) {
   }
   h[s * QS + q] = x;
 }
-
 // Candidate tiles go global -> registers -> LDS (one buffer, two barriers per
 // tile) in whole fragment blocks per wave; the amap entries of a tile are read
 // one tile ahead.  Lanes hold their query fragments in registers for the
@@ -535,16 +536,6 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
 }
 
 // Pass 2: one wave per query.  LDS heap per wave: k doubles + k ints.
-template <typename F>
-__device__ void wave_rerank_each(const Bf16Args &a, int j, F &&visit);
-
-__device__ __forceinline__ double wave_sum(double v) {
-  // Fixed tree: lane 0 ends with the sum, then broadcast; the same bits on
-  // every call for the same inputs.
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
-  return __shfl(v, 0, 64);
-}
-
 __device__ void heap_sift_d(double *hd, int *hi, int s, int k, double xd, int xi) {
   while (true) {
     int c = 2 * s + 1;
@@ -565,10 +556,24 @@ __device__ void heap_sift_d(double *hd, int *hi, int s, int k, double xd, int xi
   hi[s] = xi;
 }
 
+// FP64 direct difference sum_p (a_p - b_p)^2 of one candidate, computed by a
+// 16-lane group: lane i of the group sums dims i, i + 16, ... in order (FMA
+// chain), then a fixed tree over the group (16 -> 1) gives the total in the
+// group's first lane.  The order is the same for every call, so a pair's
+// distance does not depend on the list, the split or the batch it comes in.
+constexpr int kRerankDims = 256;  // d limit of the BF16 form
+constexpr int kDimsPerLane = kRerankDims / 16;
+
+__device__ __forceinline__ double group16_sum(double v) {
+  for (int off = 8; off >= 1; off >>= 1) v += __shfl_down(v, off, 16);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int k = a.k;
+  const int k = a.k, d = a.d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int g = lane >> 4, gl = lane & 15;
   const int j = blockIdx.x * 4 + w;
   double *hd = (double *)smem + (size_t)w * k;
   int *hi = (int *)(smem + al16r(sizeof(double) * 4 * (size_t)k)) + (size_t)w * k;
@@ -583,7 +588,14 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  const float *xq = a.XB + (size_t)a.bmap[j] * a.d;
+  // This lane's query dims gl, gl + 16, ... (the same in every group).
+  const float *xq = a.XB + (size_t)a.bmap[j] * d;
+  double qv[kDimsPerLane];
+#pragma unroll
+  for (int i = 0; i < kDimsPerLane; ++i) {
+    const int p = gl + 16 * i;
+    qv[i] = p < d ? (double)xq[p] : 0.0;
+  }
   // Every split's U bounds the final k-th distance; prune with the smallest.
   float Uf = a.U[j];
   for (int sp = 1; sp < a.nsplit; ++sp) Uf = fminf(Uf, a.U[(size_t)sp * a.n + j]);
@@ -596,29 +608,66 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
     const bool over = cnt > a.cap;
     const int cbeg = sp * a.cps;
     const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : cnt;
-    for (int e = 0; e < nscan; ++e) {
-      int pos;
-      if (over) {
-        pos = cbeg + e;
-      } else {
-        const int2 en = a.lst[sj * a.cap + e];
-        if (__int_as_float(en.y) > Ulim) continue;
-        pos = en.x;
+    for (int base = 0; base < nscan; base += 64) {
+      // One list entry (or, for an overflowed list, one candidate) per lane.
+      const int e = base + lane;
+      int pos = 0;
+      bool keep = e < nscan;
+      if (keep) {
+        if (over) {
+          pos = cbeg + e;
+        } else {
+          const int2 en = a.lst[sj * a.cap + e];
+          pos = en.x;
+          keep = __int_as_float(en.y) <= Ulim;
+        }
       }
-      const int id = a.amap[pos];
-      const float *xa = a.XA + (size_t)id * a.d;
-      double s = 0.0;
-      for (int p = lane; p < a.d; p += 64) {
-        const double t = (double)xa[p] - (double)xq[p];
-        s = __builtin_fma(t, t, s);
-      }
-      const double dist = wave_sum(s);
-      if (cless_d(dist, id, rd, ri)) {
-        if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, id);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        rd = hd[0];
-        ri = hi[0];
+      unsigned long long live = __ballot(keep);
+      while (live) {
+        // Up to four kept entries: group g takes the g-th lowest set bit.
+        int src[4];
+        int nb = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          src[t] = live ? __builtin_ctzll(live) : 0;
+          if (live) {
+            live &= live - 1;
+            ++nb;
+          }
+        }
+        int my = src[0];
+#pragma unroll
+        for (int t = 1; t < 4; ++t)
+          if (g == t) my = src[t];
+        const int mpos = __shfl(pos, my);
+        const bool act = g < nb;
+        const int id = act ? a.amap[mpos] : 0;
+        double s = 0.0;
+        if (act) {
+          const float *xa = a.XA + (size_t)id * d;
+#pragma unroll
+          for (int i = 0; i < kDimsPerLane; ++i) {
+            const int p = gl + 16 * i;
+            if (p < d) {
+              const double t = (double)xa[p] - qv[i];
+              s = __builtin_fma(t, t, s);
+            }
+          }
+        }
+        s = group16_sum(s);
+        // Apply the group results in group order (canonical selection does
+        // not depend on it).
+        for (int t = 0; t < nb; ++t) {
+          const double dist = __shfl(s, 16 * t);
+          const int cid = __shfl(id, 16 * t);
+          if (cless_d(dist, cid, rd, ri)) {
+            if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, cid);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            rd = hd[0];
+            ri = hi[0];
+          }
+        }
       }
     }
   }
