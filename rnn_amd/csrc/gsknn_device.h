@@ -130,11 +130,12 @@ __device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, i
 template <typename T>
 struct ExactLayout {
   size_t hd, hi, sq, sc, cn, ci, bytes;
-  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT) {
+  // qreg: query rows live in registers (no LDS copy).
+  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT, bool qreg = false) {
     hd = 0;
     hi = al16(hd + sizeof(T) * (size_t)k * QPB);
     sq = al16(hi + sizeof(int) * (size_t)k * QPB);
-    sc = al16(sq + sizeof(T) * (size_t)QPB * (d + 1));
+    sc = al16(sq + (qreg ? 0 : sizeof(T) * (size_t)QPB * (d + 1)));
     cn = al16(sc + sizeof(T) * (size_t)d * CT);
     ci = al16(cn + sizeof(T) * (size_t)CT);
     bytes = al16(ci + sizeof(int) * (size_t)CT);
@@ -143,11 +144,15 @@ struct ExactLayout {
 
 // One workgroup's share of a launch: queries [qblock*QPB, +QPB) against the
 // candidates of split `split`.  Every thread of the workgroup must call it.
-template <typename T, int QPB>
+// DQ > 0 (requires d <= DQ): each thread keeps its query row in registers and
+// scores 8 candidates per pass; DQ = 0 reads the row from LDS.  Either way a
+// pair's dot product is summed in p order with products and sums rounded
+// separately, the oracle's arithmetic.
+template <typename T, int QPB, int DQ = 0>
 __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
                             unsigned char *smem) {
   const int k = a.k, d = a.d;
-  const ExactLayout<T> L(k, QPB, d, CT);
+  const ExactLayout<T> L(k, QPB, d, CT, DQ > 0);
   T *hd = (T *)(smem + L.hd);
   int *hi = (int *)(smem + L.hi);
   T *sq = (T *)(smem + L.sq);
@@ -165,9 +170,15 @@ __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
   T rd = T(0);
   int ri = 0;
   T *myq = sq + (size_t)q * (d + 1);
+  T qreg[DQ > 0 ? DQ : 1];
   if (valid) {
     const T *xq = a.XB + (size_t)a.bmap[j] * d;
-    for (int p = 0; p < d; ++p) myq[p] = xq[p];
+    if (DQ > 0) {
+#pragma unroll
+      for (int p = 0; p < (DQ > 0 ? DQ : 1); ++p) qreg[p] = p < d ? xq[p] : T(0);
+    } else {
+      for (int p = 0; p < d; ++p) myq[p] = xq[p];
+    }
     qn = a.XB2[a.bmap[j]];
     heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
     rd = hd[q];
@@ -200,6 +211,32 @@ __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
     __syncthreads();
     if (!valid) continue;
     int c = 0;
+    if constexpr (DQ > 0) {
+      for (; c + 8 <= cnt; c += 8) {
+        T s[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] = T(0);
+#pragma unroll
+        for (int p = 0; p < DQ; ++p) {
+          if (p < d) {
+            const T qp = qreg[p];
+            const T *cp = sc + p * CT + c;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s[u] = xadd(s[u], xmul(cp[u], qp));
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) consider(s[u], c + u);
+      }
+      for (; c < cnt; ++c) {
+        T s0 = T(0);
+#pragma unroll
+        for (int p = 0; p < DQ; ++p)
+          if (p < d) s0 = xadd(s0, xmul(sc[p * CT + c], qreg[p]));
+        consider(s0, c);
+      }
+      continue;
+    }
     for (; c + 4 <= cnt; c += 4) {
       T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
       for (int p = 0; p < d; ++p) {
@@ -224,10 +261,13 @@ __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
   if (valid) heap_store(a, hd, hi, QPB, q, j, split);
 }
 
-template <typename T, int QPB>
+// Query rows in registers when d <= kExactDQ.
+constexpr int kExactDQ = 32;
+
+template <typename T, int QPB, int DQ>
 __global__ __launch_bounds__(QPB) void knn_exact_kernel(KnnArgs<T> a, int CT) {
   extern __shared__ __align__(16) unsigned char smem[];
-  exact_block<T, QPB>(a, CT, blockIdx.x, blockIdx.y, smem);
+  exact_block<T, QPB, DQ>(a, CT, blockIdx.x, blockIdx.y, smem);
 }
 
 

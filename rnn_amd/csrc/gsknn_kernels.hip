@@ -757,8 +757,9 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     p.ct = exact_ct(d, elem);
     p.qpb = 0;
     for (int qpb : {64, 32, 16}) {
-      const size_t b = f64 ? ExactLayout<double>(k, qpb, d, p.ct).bytes
-                           : ExactLayout<float>(k, qpb, d, p.ct).bytes;
+      const bool qreg = d <= kExactDQ;
+      const size_t b = f64 ? ExactLayout<double>(k, qpb, d, p.ct, qreg).bytes
+                           : ExactLayout<float>(k, qpb, d, p.ct, qreg).bytes;
       if (b <= kLdsMax) {
         p.qpb = qpb;
         p.lds = b;
@@ -826,7 +827,8 @@ hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   hipError_t e = hipSuccess;
 #define GSKNN_EXACT_CASE(QPB_)                                                \
   if (p.qpb == QPB_) {                                                        \
-    auto kern = knn_exact_kernel<T, QPB_>;                                    \
+    auto kern = a.d <= kExactDQ ? knn_exact_kernel<T, QPB_, kExactDQ>         \
+                                : knn_exact_kernel<T, QPB_, 0>;               \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(QPB_), p.lds, s, a, p.ct);            \
     return hipGetLastError();                                                 \

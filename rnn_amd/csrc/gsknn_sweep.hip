@@ -18,7 +18,7 @@
 
 namespace gsknn {
 
-template <typename T, int QPB>
+template <typename T, int QPB, int DQ>
 __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, const int4 *items,
                                                              const int *nitems, int CT) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -33,14 +33,14 @@ __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, co
     a.hrow = base.amap + w.x;
     a.nsplit = 1;
     a.cps = w.y;
-    exact_block<T, QPB>(a, CT, w.z / QPB, 0, smem);
+    exact_block<T, QPB, DQ>(a, CT, w.z / QPB, 0, smem);
     __syncthreads();  // LDS is reused by the next item
   }
 }
 
 template <typename T>
 size_t leaf_sweep_lds(int d, int k, int qpb, int ct) {
-  return ExactLayout<T>(k, qpb, d, ct).bytes;
+  return ExactLayout<T>(k, qpb, d, ct, d <= kExactDQ).bytes;
 }
 
 template <typename T>
@@ -58,7 +58,8 @@ hipError_t launch_leaf_sweep_t(int d, int k, const T *X, const T *X2, const int 
   hipError_t e = hipSuccess;
 #define GSKNN_LEAF_CASE(QPB_)                                                             \
   if (qpb == QPB_) {                                                                      \
-    auto kern = knn_leaf_sweep_kernel<T, QPB_>;                                           \
+    auto kern = d <= kExactDQ ? knn_leaf_sweep_kernel<T, QPB_, kExactDQ>                  \
+                              : knn_leaf_sweep_kernel<T, QPB_, 0>;                        \
     e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             (int)lds);                                                    \
     if (e != hipSuccess) return e;                                                        \
