@@ -115,6 +115,20 @@ int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
 int gsknn_rows_ascending_ddev(int n, int k, double *D, int *I, int ldk, void *stream);
 int gsknn_rows_ascending_sdev(int n, int k, float *D, int *I, int ldk, void *stream);
 
+/* One level of the randomized-projection tree of the all-NN sweep (no
+ * counterpart in the reference library, whose tree codes are outside it):
+ * element i < total of the nnodes nodes being split (node n holds perm
+ * positions [starts[n], starts[n] + size), elements numbered from offs[n],
+ * offs ascending) gets its position pos[i] and the key
+ * keys[i] = n << (32 + id_bits) | order(projection on dirs[n*d..]) << id_bits
+ * | point id, with node index, 32 order bits and id fitting in 63 bits.
+ * Sorting the keys orders each node by (projection, id). */
+int gsknn_tree_level_keys_dev(long long total, int nnodes, int d, int id_bits,
+                              const float *X, const long long *perm,
+                              const long long *starts, const long long *offs,
+                              const float *dirs, long long *pos, long long *keys,
+                              void *stream);
+
 /* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
 int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);
 int gsknn_ssqnorm_dev(int N, int d, const float *X, float *X2, void *stream);

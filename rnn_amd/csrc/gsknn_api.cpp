@@ -157,6 +157,22 @@ extern "C" int gsknn_rows_ascending_sdev(int n, int k, float *D, int *I, int ldk
   return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_rows_ascending_sdev");
 }
 
+extern "C" int gsknn_tree_level_keys_dev(long long total, int nnodes, int d, int id_bits,
+                                         const float *X, const long long *perm,
+                                         const long long *starts, const long long *offs,
+                                         const float *dirs, long long *pos, long long *keys,
+                                         void *stream) {
+  // node index, 32 order bits and the id share 63 bits
+  const int node_bits = 63 - 32 - id_bits;
+  if (total < 0 || nnodes < 0 || d <= 0 || id_bits <= 0 || (total > 0 && nnodes == 0) ||
+      node_bits <= 0 || (nnodes > 0 && ((long long)(nnodes - 1) >> node_bits) != 0))
+    return set_err(GSKNN_EINVAL, "gsknn_tree_level_keys_dev: total=%lld nnodes=%d d=%d id_bits=%d",
+                   total, nnodes, d, id_bits);
+  hipError_t e = launch_tree_level_keys(total, nnodes, d, id_bits, X, perm, starts, offs, dirs,
+                                        pos, keys, (hipStream_t)stream);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, "gsknn_tree_level_keys_dev");
+}
+
 extern "C" size_t gsknn_bf16_prep_bytes(int N, int d) {
   return (N > 0 && d > 0) ? bf16_prep_bytes(N, d) : 0;
 }

@@ -73,6 +73,12 @@ hipError_t launch_leaf_sweep_f64(int d, int k, const double *X, const double *X2
                                  const int *items, const int *nitems, double *D, int *I, int ldk,
                                  int qpb, int ct, int grid, hipStream_t s);
 size_t leaf_sweep_lds_bytes(int elem, int d, int k, int qpb, int ct);
+// One tree level's positions and sort keys (gsknn_sweep.hip).
+hipError_t launch_tree_level_keys(long long total, int nnodes, int d, int id_bits,
+                                  const float *X, const long long *perm,
+                                  const long long *starts, const long long *offs,
+                                  const float *dirs, long long *pos, long long *keys,
+                                  hipStream_t s);
 
 // BF16 candidate pass + FP64 re-rank (gsknn_rerank.hip).
 size_t bf16_prep_bytes(int N, int d);

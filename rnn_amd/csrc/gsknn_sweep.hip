@@ -38,6 +38,52 @@ __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, co
   }
 }
 
+// One level of the randomized-projection tree: for element i of the nodes
+// being split (node n holds perm[starts[n], starts[n] + sizes[n]) and its
+// elements are numbered from offs[n]), the position in perm and the sort key
+// (n, projection order, point id); sorting the keys orders every node by
+// (projection, id), ties by id, so the median split is deterministic.  The
+// projection is sum_p x_p dir_p in p order, products and sums rounded
+// separately; its FP32 bits map to an unsigned order (negative values
+// reversed).
+__global__ void tree_level_keys_kernel(long long total, int nnodes, int d, int id_bits,
+                                       const float *X, const long long *perm,
+                                       const long long *starts, const long long *offs,
+                                       const float *dirs, long long *pos_out,
+                                       long long *key_out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = nnodes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const long long pos = starts[lo] + (i - offs[lo]);
+  const long long pt = perm[pos];
+  const float *x = X + pt * d;
+  const float *dv = dirs + (size_t)lo * d;
+  float s = 0.0f;
+  for (int p = 0; p < d; ++p) s = __fadd_rn(s, __fmul_rn(x[p], dv[p]));
+  int b = __float_as_int(s);
+  if (b < 0) b ^= 0x7FFFFFFF;
+  const long long ord = (long long)b + (1LL << 31);
+  pos_out[i] = pos;
+  key_out[i] = ((long long)lo << (32 + id_bits)) | (ord << id_bits) | pt;
+}
+
+hipError_t launch_tree_level_keys(long long total, int nnodes, int d, int id_bits,
+                                  const float *X, const long long *perm,
+                                  const long long *starts, const long long *offs,
+                                  const float *dirs, long long *pos, long long *keys,
+                                  hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const long long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(tree_level_keys_kernel, dim3((unsigned)blocks), dim3(256), 0, s, total,
+                     nnodes, d, id_bits, X, perm, starts, offs, dirs, pos, keys);
+  return hipGetLastError();
+}
+
 template <typename T>
 size_t leaf_sweep_lds(int d, int k, int qpb, int ct) {
   return ExactLayout<T>(k, qpb, d, ct, d <= kExactDQ).bytes;

@@ -83,6 +83,7 @@ def _declare(L):
     L.gsknn_bf16_knn_dev.argtypes = [i32] * 5 + [vp] * 3 + [i32] + [vp] * 5 + [i32, vp, vp, sz, vp, vp]
     L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
+    L.gsknn_tree_level_keys_dev.argtypes = [ctypes.c_longlong, i32, i32, i32] + [vp] * 8
     L.gsknn_rows_ascending_ddev.argtypes = [i32, i32, vp, vp, i32, vp]
     L.gsknn_rows_ascending_sdev.argtypes = [i32, i32, vp, vp, i32, vp]
     for name in ("dgsknn_var1", "sgsknn_var1"):
@@ -246,6 +247,15 @@ def work_bytes_s(m, n, d, k, mode=MODE_AUTO) -> int:
 def sqnorm_dev(X, X2, stream=None):
     fn = lib().gsknn_dsqnorm_dev if X.element_size() == 8 else lib().gsknn_ssqnorm_dev
     rc = fn(X.shape[0], X.shape[1], _dp(X), _dp(X2), _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+def tree_level_keys_dev(X, perm, starts, offs, dirs, total, id_bits, pos, keys, stream=None):
+    """One RP-tree level on the device (see gsknn_tree_level_keys_dev)."""
+    rc = lib().gsknn_tree_level_keys_dev(int(total), starts.numel(), X.shape[1], id_bits, _dp(X),
+                                         _dp(perm), _dp(starts), _dp(offs), _dp(dirs), _dp(pos),
+                                         _dp(keys), _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
 

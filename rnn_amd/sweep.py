@@ -9,6 +9,10 @@ gsknn with amap = bmap = the leaf's point ids, merging into global heap rows
 (frame/gsknn.c:1138-1184).  Heaps carry over between iterations; a neighbour
 found again in a later iteration is kept twice, as the reference's heap would.
 
+On the device each tree level is one fused kernel of libgsknn.so
+(gsknn_tree_level_keys_dev: node lookup, projection and packed sort key per
+point) and one sort of the keys.
+
 The leaves of one iteration run in ONE persistent launch of the leaf-sweep
 kernel (libgsknn.so: gsknn_sweep_leaves_[sd]dev) over a device work list;
 that launch is captured once in a hipGraph and replayed for every iteration
@@ -43,50 +47,80 @@ def build_tree(X, iteration, seed, leaf_lo=256, leaf_hi=4096):
     """One tree over the rows of device tensor X (N, d), float32.
 
     Returns (perm, leaves): perm an int32 device tensor of point ids grouped
-    by leaf, leaves a list of (start, size) in perm order.
+    by leaf, leaves a list of (start, size) in perm order.  Node bookkeeping
+    is vectorized (numpy on the host, one pass of device work per level).
     """
     import torch
     dev = X.device
     N, d = X.shape
     g = np.random.Generator(np.random.PCG64([seed, iteration]))
     perm = torch.arange(N, device=dev, dtype=torch.int64)
-    nodes = [(0, N)]
-    leaves = []
-    while nodes:
-        thr = np.exp(g.uniform(np.log(leaf_lo), np.log(leaf_hi), size=len(nodes)))
-        split = []
-        for (s, n), t in zip(nodes, thr):
-            (split if (n > t and n >= 2) else leaves).append((s, n))
-        if not split:
+    n_start = np.zeros(1, dtype=np.int64)
+    n_size = np.full(1, N, dtype=np.int64)
+    leaf_parts = []
+    while n_start.size:
+        thr = np.exp(g.uniform(np.log(leaf_lo), np.log(leaf_hi), size=n_start.size))
+        split = (n_size > thr) & (n_size >= 2)
+        leaf_parts.append(np.stack([n_start[~split], n_size[~split]], axis=1))
+        if not split.any():
             break
-        dirs = g.standard_normal((len(split), d))
+        starts, sizes = n_start[split], n_size[split]
+        dirs = g.standard_normal((starts.size, d))
         dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
         dirs_t = torch.from_numpy(dirs.astype(np.float32)).to(dev)
-        starts = torch.tensor([s for s, _ in split], device=dev, dtype=torch.int64)
-        sizes = torch.tensor([n for _, n in split], device=dev, dtype=torch.int64)
-        node = torch.repeat_interleave(torch.arange(len(split), device=dev), sizes)
-        offs = torch.cumsum(sizes, 0) - sizes
-        pos = starts[node] + (torch.arange(node.numel(), device=dev) - offs[node])
+        starts_t = torch.from_numpy(starts).to(dev)
+        total = int(sizes.sum())
+        # order inside each node by (projection, point id): one sort of the
+        # packed key (node, projection, id) when it fits in 63 bits, else a
+        # sort by id followed by a stable sort by (node, projection)
+        id_bits = max(1, int(N - 1).bit_length())
+        node_bits = max(1, int(starts.size - 1).bit_length())
+        fits = node_bits + 32 + id_bits <= 63
+        if fits and X.is_cuda:
+            # positions and keys from one fused kernel (libgsknn.so)
+            offs_t = torch.from_numpy(np.cumsum(sizes) - sizes).to(dev)
+            pos = torch.empty(total, dtype=torch.int64, device=dev)
+            key = torch.empty(total, dtype=torch.int64, device=dev)
+            gsknn.tree_level_keys_dev(X, perm, starts_t, offs_t, dirs_t, total, id_bits, pos, key)
+            perm[pos] = torch.sort(key).values & ((1 << id_bits) - 1)
+            half = sizes // 2
+            n_start = np.stack([starts, starts + half], axis=1).reshape(-1)
+            n_size = np.stack([half, sizes - half], axis=1).reshape(-1)
+            continue
+        sizes_t = torch.from_numpy(sizes).to(dev)
+        node = torch.repeat_interleave(torch.arange(starts.size, device=dev), sizes_t,
+                                       output_size=total)
+        offs = torch.cumsum(sizes_t, 0) - sizes_t
+        pos = starts_t[node] + (torch.arange(node.numel(), device=dev) - offs[node])
         pts = perm[pos]
         proj = (X[pts] * dirs_t[node]).sum(dim=1)
-        # order inside each node by (projection, point id)
-        o1 = torch.argsort(node * N + pts)
-        node1, pts1, proj1 = node[o1], pts[o1], proj[o1]
-        o2 = torch.sort(node1 * (1 << 33) + _orderable(proj1), stable=True).indices
-        perm[pos] = pts1[o2]
-        nodes = []
-        for s, n in split:
-            h = n // 2
-            nodes += [(s, h), (s + h, n - h)]
-    leaves.sort()
-    return perm.to(torch.int32), leaves
+        if fits:
+            key = (node << (32 + id_bits)) | (_orderable(proj) << id_bits) | pts
+            perm[pos] = pts[torch.argsort(key)]
+        else:
+            o1 = torch.argsort(node * N + pts)
+            node1, pts1, proj1 = node[o1], pts[o1], proj[o1]
+            o2 = torch.sort(node1 * (1 << 33) + _orderable(proj1), stable=True).indices
+            perm[pos] = pts1[o2]
+        half = sizes // 2
+        n_start = np.stack([starts, starts + half], axis=1).reshape(-1)
+        n_size = np.stack([half, sizes - half], axis=1).reshape(-1)
+    leaves = np.concatenate(leaf_parts) if leaf_parts else np.zeros((0, 2), np.int64)
+    leaves = leaves[np.argsort(leaves[:, 0], kind="stable")]
+    return perm.to(torch.int32), [(int(s_), int(n_)) for s_, n_ in leaves]
 
 
 def make_items(leaves, qpb):
-    """Work list [nitems][4] = (start, size, first query, 0), largest leaves first."""
-    items = [(s, n, q0, 0) for s, n in leaves for q0 in range(0, n, qpb)]
-    items.sort(key=lambda t: -t[1])
-    return np.asarray(items, dtype=np.int32).reshape(-1, 4)
+    """Work list [nitems][4] = (start, size, first query, 0), largest leaves
+    first (ties in leaf order, query blocks ascending)."""
+    lv = np.asarray(leaves, dtype=np.int64).reshape(-1, 2)
+    nblk = (lv[:, 1] + qpb - 1) // qpb
+    leaf = np.repeat(np.arange(lv.shape[0]), nblk)
+    first = np.cumsum(nblk) - nblk
+    q0 = (np.arange(leaf.size) - first[leaf]) * qpb
+    items = np.stack([lv[leaf, 0], lv[leaf, 1], q0, np.zeros_like(q0)], axis=1)
+    items = items[np.argsort(-items[:, 1], kind="stable")]
+    return items.astype(np.int32).reshape(-1, 4)
 
 
 def _cache_path():

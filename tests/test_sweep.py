@@ -93,3 +93,35 @@ def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
         np.testing.assert_array_equal(a[1], b[1])
         assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
     assert sw.graph is not None
+
+
+@pytest.mark.gpu
+def test_device_tree_partition_and_median():
+    # The device build (fused per-level key kernel) partitions the points
+    # into bounded leaves, and its root split is the median of the
+    # projection summed in p order in FP32, ties by point id.
+    import torch
+    dev = torch.device("cuda")
+    X = _cpu_points(N=6000, d=8)
+    print("[gpu] device tree N=6000 d=8", flush=True)
+    perm, leaves = sweep.build_tree(X.to(dev), 0, seed=3, leaf_lo=64, leaf_hi=512)
+    p = perm.cpu().numpy()
+    assert np.array_equal(np.sort(p), np.arange(X.shape[0]))
+    sizes = [n for _, n in leaves]
+    assert sum(sizes) == X.shape[0] and max(sizes) <= 512 and min(sizes) >= 32
+    assert all(s1 + n1 == s2 for (s1, n1), (s2, _) in zip(leaves, leaves[1:]))
+    g = np.random.Generator(np.random.PCG64(1))
+    Y = g.standard_normal((200, 3)).astype(np.float32)
+    Y[50:60] = Y[40]  # exact ties in the projection
+    perm, leaves = sweep.build_tree(torch.from_numpy(Y).to(dev), 0, seed=9, leaf_lo=150,
+                                    leaf_hi=199)
+    assert leaves == [(0, 100), (100, 100)]
+    gg = np.random.Generator(np.random.PCG64([9, 0]))
+    gg.uniform(np.log(150), np.log(199), size=1)
+    direc = gg.standard_normal((1, 3))
+    direc = (direc / np.linalg.norm(direc)).astype(np.float32)[0]
+    proj = np.zeros(200, dtype=np.float32)
+    for q in range(3):
+        proj = (proj + Y[:, q] * direc[q]).astype(np.float32)
+    order = sorted(range(200), key=lambda i: (proj[i], i))
+    assert list(perm.cpu().numpy()[:100]) == sorted(order[:100], key=lambda i: (proj[i], i))
