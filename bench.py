@@ -34,27 +34,56 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, seconds_target=12.0):
-    """Oracle (C restatement, OpenMP) on all candidates x a slice of queries."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, seconds_per_run=2.0):
+    """Oracle (C restatement, OpenMP) on all candidates x a slice of queries.
+
+    The slice is sized so one run takes about `seconds_per_run`; the figure is
+    the median of 3 runs, at the host's thread share (at most 16) and again at
+    1 thread."""
     import numpy as np
 
     from oracle import oracle
-    threads = min(16, os.cpu_count() or 1)
-    nq = 64
-    while True:
+
+    def run(nq, threads):
         D = np.full((nq, p.k), np.inf)
         I = np.full((nq, p.k), -1, dtype=np.int32)
         bm = p.bmap[:nq].copy()
         t0 = time.perf_counter()
         oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, D, I, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt > seconds_target / 4 or nq >= p.n:
-            break
-        nq = min(p.n, int(nq * max(2.0, seconds_target / 4 / max(dt, 1e-3))))
-    pairs = p.m * nq
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+        return time.perf_counter() - t0
+
+    def measure(threads):
+        nq = 16
+        while True:
+            dt = run(nq, threads)
+            if dt > seconds_per_run / 2 or nq >= p.n:
+                break
+            nq = min(p.n, int(nq * max(2.0, seconds_per_run / max(dt, 1e-3))))
+        times = sorted(run(nq, threads) for _ in range(3))
+        return nq, times[1]
+
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, nproc or 1))
+    nq, dt = measure(threads)
+    nq1, dt1 = measure(1)
+    return {"value": p.m * nq / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"{nq} queries x {p.m} candidates (d={p.d}, k={p.k}) of the same call, "
-                      f"{dt:.2f} s, oracle/gsknn_oracle.c -O3 -fopenmp"}
+                      f"median of 3 runs of {dt:.2f} s, oracle/gsknn_oracle.c -O3 -fopenmp",
+            "single_thread": {"value": p.m * nq1 / dt1, "unit": "pairs/s", "cores": 1,
+                              "sample": f"{nq1} queries x {p.m} candidates, median of 3 runs "
+                                        f"of {dt1:.2f} s"},
+            "nproc": nproc, "cpu_model": _cpu_model()}
 
 
 def copy_calibration(torch, dev, args):
