@@ -13,6 +13,11 @@
  *                                            frame/sgsknn_ref_impl.h:41-103
  *   - heapSelect_[sd] + HeapAdjust_[sd]      frame/gsknn_heap.c:52-162
  *   - heap row stride = r (k) for k <= 512   frame/gsknn.c:930
+ *   - L1 norm (KNN_1NORM kernels, blocked
+ *     sum of |a_p - b_p|)                    frame/gsknn.c:996-1086,
+ *                                            kernels/x86_64/sandybridge/
+ *                                            rnn_rank_k_abs_int_d8x4.c,
+ *                                            rnn_r_1norm_int_d8x4_row.c
  *
  * Two deliberate differences from the reference oracle:
  *   1. Blocking.  The reference materialises the m x n matrix Cs
@@ -35,6 +40,10 @@
  * the committed fixtures in tests/golden/ made from it; brute force in long
  * double on small inputs; integer lattices with exact ties; chunking and
  * permutation invariants; d = 1 reducing to sorting |q - r|.
+ * L1 pins (tests/test_oracle_l1.py): heap arrays bit-identical to the
+ * reference's own L1 kernels reached through its kernel tables
+ * (oracle/_ref/libgsknn_opt.so, live and as fixtures), an unblocked sum
+ * failing that fixture, long-double brute force, integer lattices, d = 1.
  */
 #include <math.h>
 #include <stdint.h>
@@ -202,6 +211,81 @@ void oracle_sgsknn(int m, int n, int d, int k, const float *XA,
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* L1 norm (knn_type KNN_1NORM).                                        */
+/* ------------------------------------------------------------------ */
+
+/* The reference registers L1 kernels at index KNN_1NORM of its FP64 kernel
+ * tables (kernels/x86_64/sandybridge/gsknn_kernel.h:125-163) but its driver
+ * always calls index 0 (frame/gsknn.c:265, :409).  Routed through the
+ * tables, dgsknn_var1 computes, per pair, |a_p - b_p| summed in p order
+ * within blocks of DKNN_KC = 256 dimensions, each block sum starting from
+ * zero and the block sums added in order (frame/gsknn.c:996-1086,
+ * rnn_rank_k_abs_int_d8x4.c:62-195 and :226-247, rnn_r_1norm_int_d8x4_row.c
+ * :46-275), then max(0, s), a no-op on these sums, and the same row heap
+ * select as the 2-norm.  dgsknn_ref_impl.h:74-85 keeps the unblocked sum as
+ * a comment; it equals this for d <= 256.  The FP32 tables hold no L1 row
+ * kernel; the FP32 form here applies the same rule in float. */
+#define ORACLE_L1_KC 256
+
+static inline double dist_l1_d(const double *q, const double *c, int d) {
+  double s = 0.0;
+  for (int p0 = 0; p0 < d; p0 += ORACLE_L1_KC) {
+    const int pe = p0 + ORACLE_L1_KC < d ? p0 + ORACLE_L1_KC : d;
+    double t = 0.0;
+    for (int p = p0; p < pe; p++) t += fabs(q[p] - c[p]);
+    s = p0 == 0 ? t : s + t;
+  }
+  return s;
+}
+
+static inline float dist_l1_s(const float *q, const float *c, int d) {
+  float s = 0.0f;
+  for (int p0 = 0; p0 < d; p0 += ORACLE_L1_KC) {
+    const int pe = p0 + ORACLE_L1_KC < d ? p0 + ORACLE_L1_KC : d;
+    float t = 0.0f;
+    for (int p = p0; p < pe; p++) t += fabsf(q[p] - c[p]);
+    s = p0 == 0 ? t : s + t;
+  }
+  return s;
+}
+
+/* Same roles and heap rows as oracle_dgsknn: m candidates (XA, amap), n
+ * queries (XB, bmap), heap row j of stride ldk; no norms. */
+void oracle_dgsknn_l1(int m, int n, int d, int k, const double *XA,
+                      const int *amap, const double *XB, const int *bmap,
+                      double *D, int *I, int ldk, int canonical) {
+  if (m == 0 || n == 0 || d == 0 || k == 0) return;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < n; j++) {
+    const double *b = XB + (size_t)bmap[j] * d;
+    double *Dj = D + (size_t)j * ldk;
+    int *Ij = I + (size_t)j * ldk;
+    if (canonical) oracle_heapify_canon_d(k, Dj, Ij);
+    for (int i = 0; i < m; i++) {
+      double x = dist_l1_d(b, XA + (size_t)amap[i] * d, d);
+      oracle_heap_select_d(1, k, &x, &amap[i], Dj, Ij, canonical);
+    }
+  }
+}
+
+void oracle_sgsknn_l1(int m, int n, int d, int k, const float *XA,
+                      const int *amap, const float *XB, const int *bmap,
+                      float *D, int *I, int ldk, int canonical) {
+  if (m == 0 || n == 0 || d == 0 || k == 0) return;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < n; j++) {
+    const float *b = XB + (size_t)bmap[j] * d;
+    float *Dj = D + (size_t)j * ldk;
+    int *Ij = I + (size_t)j * ldk;
+    if (canonical) oracle_heapify_canon_s(k, Dj, Ij);
+    for (int i = 0; i < m; i++) {
+      float x = dist_l1_s(b, XA + (size_t)amap[i] * d, d);
+      oracle_heap_select_s(1, k, &x, &amap[i], Dj, Ij, canonical);
+    }
+  }
+}
+
 /* The reference's own entry points (gsknn_ref.h:40-68), compat mode, row
  * stride r.  Argument k is the dimension and r the neighbour count, as in
  * the reference. */
@@ -349,6 +433,30 @@ void oracle_brute_d(int m, int n, int d, int k, const double *XA,
         long double t = (long double)a[p] - (long double)b[p];
         s += t * t;
       }
+      buf[i].d = (double)s;
+      buf[i].i = amap[i];
+    }
+    qsort(buf, (size_t)m, sizeof(pair_d), cmp_pair_d);
+    for (int t = 0; t < k && t < m; t++) {
+      D[(size_t)j * ldk + t] = buf[t].d;
+      I[(size_t)j * ldk + t] = buf[t].i;
+    }
+  }
+  free(buf);
+}
+
+/* L1 brute force in long double (sum of |a_p - b_p| with one final
+ * rounding for d < 2^11), full sort by (distance, index); tiny inputs. */
+void oracle_brute_l1_d(int m, int n, int d, int k, const double *XA,
+                       const int *amap, const double *XB, const int *bmap,
+                       double *D, int *I, int ldk) {
+  pair_d *buf = (pair_d *)malloc(sizeof(pair_d) * (size_t)(m > 0 ? m : 1));
+  for (int j = 0; j < n; j++) {
+    const double *b = XB + (size_t)bmap[j] * d;
+    for (int i = 0; i < m; i++) {
+      const double *a = XA + (size_t)amap[i] * d;
+      long double s = 0.0L;
+      for (int p = 0; p < d; p++) s += fabsl((long double)a[p] - (long double)b[p]);
       buf[i].d = (double)s;
       buf[i].i = amap[i];
     }

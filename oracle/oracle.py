@@ -3,7 +3,8 @@
 Loads ``oracle/liboracle.so`` (this repository's C restatement of the
 reference's ``dgsknn_ref``/``sgsknn_ref``, frame/gsknn_ref.c:63-162) and, when
 built, ``oracle/_ref/libgsknn_ref.so`` (the reference's own oracle sources
-compiled by ``oracle/Makefile``).  Only tests, ``__graft_entry__.smoke()`` and
+compiled by ``oracle/Makefile``) and ``oracle/_ref/libgsknn_opt.so`` (its
+optimized library, whose kernel tables reach the L1 kernels).  Only tests, ``__graft_entry__.smoke()`` and
 the cpu_baseline leg of ``bench.py`` import this module; the product library
 never does.
 """
@@ -18,9 +19,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libgsknn_ref.so")
+REF_OPT_PATH = os.path.join(HERE, "_ref", "libgsknn_opt.so")
 
 _lib = None
 _ref = None
+_ref_opt = None
 
 
 def build(quiet: bool = True) -> None:
@@ -47,6 +50,15 @@ def ref_lib():
     if _ref is None and os.path.exists(REF_PATH):
         _ref = ctypes.CDLL(REF_PATH)
     return _ref
+
+
+def ref_opt_lib():
+    """The reference's optimized library (frame/gsknn.c and its x86-64 AVX
+    kernels) compiled from its own sources, or None."""
+    global _ref_opt
+    if _ref_opt is None and os.path.exists(REF_OPT_PATH):
+        _ref_opt = ctypes.CDLL(REF_OPT_PATH)
+    return _ref_opt
 
 
 def _p(a):
@@ -79,6 +91,61 @@ def gsknn(XA, XA2, amap, XB, XB2, bmap, D, I, canonical=False, threads=None):
         ctypes.c_int(m), ctypes.c_int(n), ctypes.c_int(d), ctypes.c_int(k),
         _p(XA), _p(XA2), _p(amap), _p(XB), _p(XB2), _p(bmap), _p(D), _p(I),
         ctypes.c_int(ldk), ctypes.c_int(1 if canonical else 0)))
+
+
+def gsknn_l1(XA, amap, XB, bmap, D, I, canonical=False, threads=None):
+    """L1 (KNN_1NORM) oracle, same roles and rows as ``gsknn``, no norms."""
+    dt = XA.dtype
+    assert XB.dtype == dt and D.dtype == dt and I.dtype == np.int32 and D.shape == I.shape
+    for a in (XA, XB, amap, bmap, D, I):
+        assert a.flags.c_contiguous
+    assert amap.dtype == np.int32 and bmap.dtype == np.int32
+    m, n, d = amap.size, bmap.size, XA.shape[1]
+    assert XB.shape[1] == d and D.shape[0] == n
+    ldk = D.shape[1]
+    fn = lib().oracle_dgsknn_l1 if dt == np.float64 else lib().oracle_sgsknn_l1
+    _with_threads(threads, lambda: fn(
+        ctypes.c_int(m), ctypes.c_int(n), ctypes.c_int(d), ctypes.c_int(ldk),
+        _p(XA), _p(amap), _p(XB), _p(bmap), _p(D), _p(I), ctypes.c_int(ldk),
+        ctypes.c_int(1 if canonical else 0)))
+
+
+def ref_opt_l1(XA, amap, XB, bmap, D, I):
+    """The reference's own FP64 L1 path: its dgsknn_var1 (frame/gsknn.c:
+    883-1103) run with kernel-table slot 0 pointed at the KNN_1NORM entries
+    (rankk_d[1], micro_d[1]; kernels/x86_64/sandybridge/gsknn_kernel.h:
+    125-163), restored afterwards.  m = amap.size candidates, n = bmap.size
+    heap rows of stride k = D.shape[1] (<= 512)."""
+    r = ref_opt_lib()
+    if r is None:
+        raise FileNotFoundError(REF_OPT_PATH)
+    m, n, d, k = amap.size, bmap.size, XA.shape[1], D.shape[1]
+    assert XA.dtype == np.float64 and XB.dtype == np.float64 and k <= 512
+    micro = (ctypes.c_void_p * 2).in_dll(r, "micro_d")
+    rankk = (ctypes.c_void_p * 2).in_dll(r, "rankk_d")
+    saved = (micro[0], rankk[0])
+    zeros_a = np.zeros(XA.shape[0])
+    zeros_b = np.zeros(XB.shape[0])
+    micro[0], rankk[0] = micro[1], rankk[1]
+    try:
+        # var1 roles: queries first (heap rows), candidates second.
+        r.dgsknn_var1(ctypes.c_int(n), ctypes.c_int(m), ctypes.c_int(d), ctypes.c_int(k),
+                      _p(XB), _p(zeros_b), _p(bmap), _p(XA), _p(zeros_a), _p(amap), _p(D), _p(I))
+    finally:
+        micro[0], rankk[0] = saved
+
+
+def brute_l1(XA, amap, XB, bmap, k):
+    """Long-double L1 brute force; rows ascending by (dist, idx)."""
+    XA = np.ascontiguousarray(XA, dtype=np.float64)
+    XB = np.ascontiguousarray(XB, dtype=np.float64)
+    m, n, d = amap.size, bmap.size, XA.shape[1]
+    kk = min(k, m)
+    D = np.full((n, kk), np.inf)
+    I = np.full((n, kk), -1, dtype=np.int32)
+    lib().oracle_brute_l1_d(ctypes.c_int(m), ctypes.c_int(n), ctypes.c_int(d), ctypes.c_int(kk),
+                            _p(XA), _p(amap), _p(XB), _p(bmap), _p(D), _p(I), ctypes.c_int(kk))
+    return D, I
 
 
 def ref_gsknn(XA, XA2, amap, XB, XB2, bmap, D, I):

@@ -12,10 +12,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
 
 
-@pytest.fixture(scope="session")
-def fixtures():
+def _load_cases(fname):
     import numpy as np
-    path = os.path.join(ROOT, "tests", "golden", "gsknn_ref_fixtures.npz")
+    path = os.path.join(ROOT, "tests", "golden", fname)
     with np.load(path, allow_pickle=False) as z:
         data = {k: z[k] for k in z.files}
     cases = {}
@@ -23,3 +22,13 @@ def fixtures():
         name, field = key.split("__")
         cases.setdefault(name, {})[field] = val
     return cases
+
+
+@pytest.fixture(scope="session")
+def fixtures():
+    return _load_cases("gsknn_ref_fixtures.npz")
+
+
+@pytest.fixture(scope="session")
+def l1_fixtures():
+    return _load_cases("gsknn_ref_l1_fixtures.npz")

@@ -1,6 +1,6 @@
 """Write the golden fixtures in this directory from the reference's own oracle.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [l2|l1]
 
 The reference ships no golden vectors (its drivers print instead of assert,
 test/test_dgsknn.c:13-104), so the fixtures are made here: seeded inputs, run
@@ -11,6 +11,11 @@ beside the inputs.  Only data is written; nothing of the reference is copied.
 The stored D/I are the raw heap arrays after each call, so the fixtures pin
 the heap layout (insertion order under the reference's replace-on-<= rule) as
 well as the neighbour sets.
+
+``l1`` writes gsknn_ref_l1_fixtures.npz from the reference's optimized
+library (oracle/_ref/libgsknn_opt.so) run through its KNN_1NORM kernel-table
+entries (oracle.ref_opt_l1): the L1 norm has no oracle form in the
+reference, so its own kernels are the source.
 """
 from __future__ import annotations
 
@@ -106,5 +111,44 @@ def main():
     print(out, os.path.getsize(out), "bytes,", len(recs), "arrays")
 
 
+def l1_case(name, X, amap, bmap, k):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    amap = np.ascontiguousarray(amap, dtype=np.int32)
+    bmap = np.ascontiguousarray(bmap, dtype=np.int32)
+    D = np.full((bmap.size, k), 1.79e308)
+    I = np.full((bmap.size, k), -1, dtype=np.int32)
+    oracle.ref_opt_l1(X, amap, X, bmap, D, I)
+    return {f"{name}__X": X, f"{name}__amap": amap, f"{name}__bmap": bmap,
+            f"{name}__D": D, f"{name}__I": I}
+
+
+def main_l1():
+    if oracle.ref_opt_lib() is None:
+        raise SystemExit("oracle/_ref/libgsknn_opt.so missing: run `make -C oracle ref`")
+    g = np.random.Generator(np.random.PCG64(77))
+    recs = {}
+    # d = 64 (one dimension block) and d = 600 (three blocks of 256, 256, 88).
+    X = g.standard_normal((160, 64))
+    recs.update(l1_case("l1_d64", X, g.permutation(160)[:140], g.permutation(160)[:40], 16))
+    X = g.standard_normal((48, 600))
+    recs.update(l1_case("l1_d600", X, g.permutation(48)[:36], g.permutation(48)[:12], 8))
+    # d = 1, and more neighbours than a 4-wide candidate group.
+    X = g.standard_normal((94, 1))
+    recs.update(l1_case("l1_d1", X, g.permutation(94)[:64], g.permutation(94)[:30], 5))
+    X = g.standard_normal((600, 5))
+    recs.update(l1_case("l1_k40", X, g.permutation(600)[:560], g.permutation(600)[:50], 40))
+    # Integer lattice: exact L1 distances with many exact ties.
+    pts = np.array([(x, y, z) for x in range(5) for y in range(5) for z in range(5)],
+                   dtype=np.float64)
+    recs.update(l1_case("l1_lattice", pts, g.permutation(125), np.arange(0, 125, 5), 10))
+    out = os.path.join(HERE, "gsknn_ref_l1_fixtures.npz")
+    np.savez_compressed(out, **recs)
+    print(out, os.path.getsize(out), "bytes,", len(recs), "arrays")
+
+
 if __name__ == "__main__":
-    main()
+    which = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if which in ("all", "l2"):
+        main()
+    if which in ("all", "l1"):
+        main_l1()
