@@ -342,12 +342,70 @@ def serving(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+FP64_VALU_OPS_PEAK = 39.3e12  # 78.6 TFLOP/s FP64 vector counts an FMA as 2
+
+
+def l1_norm(torch, dev, args):
+    """L1 (KNN_1NORM) kNN on the config-2 point set: N = 131072, d = 64,
+    m = n = 65536, k = 32, empty heaps; FP64, then the same points in FP32.
+    Every pair costs 2 VALU operations per dimension (|q - c| and the sum
+    step), so the roofline is the FP64 VALU rate for non-FMA operations."""
+    import numpy as np
+
+    from rnn_amd import datasets as ds
+    from rnn_amd import gsknn
+
+    p = ds.config2()
+    stream = torch.cuda.current_stream()
+    am = torch.from_numpy(p.amap).to(dev)
+    bm = torch.from_numpy(p.bmap).to(dev)
+    pairs = float(p.m) * p.n
+    ops = 2.0 * p.d * pairs
+    out = {"workload": "l1", "m": p.m, "n": p.n, "d": p.d, "k": p.k}
+    res = {}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        X = torch.from_numpy(p.XA).to(dev).to(dt)
+        D = torch.empty((p.n, p.k), dtype=dt, device=dev)
+        I = torch.empty((p.n, p.k), dtype=torch.int32, device=dev)
+        wb = gsknn.work_bytes_l1(p.m, p.n, p.d, p.k, X.element_size())
+        work = torch.empty(max(16, wb), dtype=torch.uint8, device=dev)
+
+        def call():
+            D.fill_(float("inf"))
+            I.fill_(-1)
+            gsknn.knn1_dev(X, am, X, bm, D, I, work=work, stream=stream)
+
+        call()
+        ms, _ = _timed(torch, stream, call, max(1, args.steps // 2))
+        res[name] = (D, I)
+        out[name] = {"ms": ms, "pairs_per_s": pairs / (ms * 1e-3),
+                     "valu_tops": ops / (ms * 1e-3) / 1e12,
+                     "frac_fp64_valu_peak": ops / (ms * 1e-3) / FP64_VALU_OPS_PEAK}
+        log(f"l1 {name} {ms:.2f} ms, {ops / (ms * 1e-3) / 1e12:.2f} TOP/s")
+    # Independent spot check (numpy) on a few FP64 rows.
+    rows = np.linspace(0, p.n - 1, 16).astype(np.int64)
+    D, I = res["fp64"]
+    Dg, Ig = D.cpu().numpy()[rows], I.cpu().numpy()[rows]
+    ok = True
+    Xc = p.XA[p.amap]
+    for r, qi in enumerate(p.bmap[rows]):
+        dn = np.abs(Xc - p.XA[qi]).sum(1)
+        ref = np.sort(np.partition(dn, p.k - 1)[:p.k])
+        ok &= bool(np.abs(np.sort(Dg[r]) - ref).max() <= 1e-12 * max(1.0, float(ref.max())))
+        ok &= bool(np.allclose(np.abs(p.XA[Ig[r]] - p.XA[qi]).sum(1), Dg[r], rtol=1e-13, atol=0))
+    out["spot_check"] = ok
+    Ds, Is = sort_rows(D.cpu().numpy(), I.cpu().numpy())
+    D32s, I32s = sort_rows(res["fp32"][0].cpu().numpy(), res["fp32"][1].cpu().numpy())
+    out["rows_equal_fp32_vs_fp64"] = float(np.mean(np.all(Is == I32s, axis=1)))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "l1", "copy"])
     ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
     ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
@@ -378,6 +436,9 @@ def main():
         return
     if args.workload == "c5":
         serving(torch, dev, args)
+        return
+    if args.workload == "l1":
+        l1_norm(torch, dev, args)
         return
     t0 = time.perf_counter()
     p = ds.config2()

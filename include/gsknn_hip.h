@@ -19,6 +19,8 @@
 
 #include <stddef.h>
 
+#include "gsknn.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -128,6 +130,41 @@ int gsknn_tree_level_keys_dev(long long total, int nnodes, int d, int id_bits,
                               const long long *starts, const long long *offs,
                               const float *dirs, long long *pos, long long *keys,
                               void *stream);
+
+/* L1-norm kNN (knn_type KNN_1NORM).  The reference registers L1 kernels at
+ * index KNN_1NORM of its FP64 kernel tables (kernels/x86_64/sandybridge/
+ * gsknn_kernel.h:125-163: rnn_rank_k_abs_int_d8x4, rnn_r_1norm_int_d8x4_row)
+ * but its dgsknn always dispatches index 0 (frame/gsknn.c:265, :409), so
+ * heap->type never selects them; these entry points are that path, driven
+ * as dgsknn_var1 drives the tables (frame/gsknn.c:996-1086): distance =
+ * sum_p |a_p - b_p|, summed in p order within blocks of 256 dimensions with
+ * the block sums added in order, then the same heap select, ties to the
+ * lower index.  No norms are read.
+ *
+ * dgsknn_1norm / sgsknn_1norm: the dgsknn / sgsknn argument convention
+ * (include/gsknn.h:90-116) without XA2/XB2 -- m candidates XA[alpha[i]], n
+ * queries XB[beta[j]], k the dimension, heap rows of stride r (r <= 512;
+ * the reference's large-r header layout has no L1 form) in heap->D (FP64)
+ * or heap->D_s (FP32) and heap->I.  Host or device pointers; errors via
+ * gsknn_last_error().  FP32 applies the same rule in float (the reference's
+ * FP32 tables hold no L1 row kernel). */
+void dgsknn_1norm(int m, int n, int k, int r, double *XA, int *alpha,
+                  double *XB, int *beta, heap_t *heap);
+void sgsknn_1norm(int m, int n, int k, int r, float *XA, int *alpha,
+                  float *XB, int *beta, heap_t *heap);
+
+/* Device-array form (conventions of gsknn_dknn_dev; `work` as there, from
+ * gsknn_[ds]knn1_work_bytes). */
+size_t gsknn_dknn1_work_bytes(int m, int n, int d, int k);
+int gsknn_dknn1_dev(int m, int n, int d, int k, const double *XA,
+                    const int *amap, const double *XB, const int *bmap,
+                    double *D, int *I, int ldk, const int *hrow, void *work,
+                    size_t work_bytes, void *stream);
+size_t gsknn_sknn1_work_bytes(int m, int n, int d, int k);
+int gsknn_sknn1_dev(int m, int n, int d, int k, const float *XA,
+                    const int *amap, const float *XB, const int *bmap,
+                    float *D, int *I, int ldk, const int *hrow, void *work,
+                    size_t work_bytes, void *stream);
 
 /* Squared norms X2[i] = sum_p X[i*d + p]^2, summed in p order. */
 int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream);

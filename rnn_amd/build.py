@@ -19,8 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgsknn.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip", "gsknn_api.cpp",
-           "gsknn_serve.cpp"]
+SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip",
+           "gsknn_l1.hip", "gsknn_api.cpp", "gsknn_serve.cpp"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",

@@ -127,6 +127,64 @@ extern "C" int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA, const
                  mode, work, work_bytes, stream);
 }
 
+template <typename T>
+static int knn1_dev(const char *fn, int m, int n, int d, int k, const T *XA, const int *amap,
+                    const T *XB, const int *bmap, T *D, int *I, int ldk, const int *hrow,
+                    void *work, size_t work_bytes, void *stream) {
+  int rc = check_sizes(fn, m, n, d, k, ldk);
+  if (rc) return rc;
+  if (m == 0 || n == 0 || d == 0 || k == 0) return GSKNN_OK;
+  if (!XA || !amap || !XB || !bmap || !D || !I) return set_err(GSKNN_EINVAL, "%s: null array", fn);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t need = knn_l1_work_bytes(m, n, d, k, (int)sizeof(T));
+  void *own = nullptr;
+  if (need > 0 && (work == nullptr || work_bytes < need)) {
+    if (work != nullptr)
+      return set_err(GSKNN_EINVAL, "%s: workspace %zu < %zu bytes", fn, work_bytes, need);
+    hipError_t e = hipMallocAsync(&own, need, s);
+    if (e != hipSuccess) return hip_err(e, "hipMallocAsync(workspace)");
+    work = own;
+    work_bytes = need;
+  }
+  KnnArgs<T> a{};
+  a.m = m; a.n = n; a.d = d; a.k = k; a.ldk = ldk;
+  a.XA = XA; a.amap = amap;
+  a.XB = XB; a.bmap = bmap;
+  a.D = D; a.I = I; a.hrow = hrow;
+  hipError_t e;
+  if constexpr (sizeof(T) == 8)
+    e = launch_knn_l1_f64(a, s, work, work_bytes);
+  else
+    e = launch_knn_l1_f32(a, s, work, work_bytes);
+  if (own) (void)hipFreeAsync(own, s);
+  if (e == hipErrorNotSupported)
+    return set_err(GSKNN_ELIMIT, "%s: no kernel configuration fits d=%d k=%d", fn, d, k);
+  if (e != hipSuccess) return hip_err(e, fn);
+  return GSKNN_OK;
+}
+
+extern "C" size_t gsknn_dknn1_work_bytes(int m, int n, int d, int k) {
+  return knn_l1_work_bytes(m, n, d, k, 8);
+}
+
+extern "C" size_t gsknn_sknn1_work_bytes(int m, int n, int d, int k) {
+  return knn_l1_work_bytes(m, n, d, k, 4);
+}
+
+extern "C" int gsknn_dknn1_dev(int m, int n, int d, int k, const double *XA, const int *amap,
+                               const double *XB, const int *bmap, double *D, int *I, int ldk,
+                               const int *hrow, void *work, size_t work_bytes, void *stream) {
+  return knn1_dev("gsknn_dknn1_dev", m, n, d, k, XA, amap, XB, bmap, D, I, ldk, hrow, work,
+                  work_bytes, stream);
+}
+
+extern "C" int gsknn_sknn1_dev(int m, int n, int d, int k, const float *XA, const int *amap,
+                               const float *XB, const int *bmap, float *D, int *I, int ldk,
+                               const int *hrow, void *work, size_t work_bytes, void *stream) {
+  return knn1_dev("gsknn_sknn1_dev", m, n, d, k, XA, amap, XB, bmap, D, I, ldk, hrow, work,
+                  work_bytes, stream);
+}
+
 extern "C" int gsknn_dsqnorm_dev(int N, int d, const double *X, double *X2, void *stream) {
   if (N < 0 || d < 0) return set_err(GSKNN_EINVAL, "gsknn_dsqnorm_dev: negative size");
   hipError_t e = launch_sqnorm_f64(N, d, X, X2, (hipStream_t)stream);
@@ -342,30 +400,33 @@ auto upload(DevBuf &b, const void *src, size_t bytes, void **dst, hipStream_t s)
 
 // Caller holds g_stage_mu.  Point tables are copied up to the largest row
 // the maps touch; one copy serves both roles when XA == XB.
+// With norms = false (L1 calls) XA2/XB2 are neither read nor staged.
 template <typename T>
 int stage_inputs(const char *fn, int m, int n, int d, const T *XA, const T *XA2,
                  const int *amap, const T *XB, const T *XB2, const int *bmap, hipStream_t s,
-                 Inputs<T> *out) {
-  if (!XA || !XA2 || !amap || !XB || !XB2 || !bmap)
+                 Inputs<T> *out, bool norms = true) {
+  if (!XA || !amap || !XB || !bmap || (norms && (!XA2 || !XB2)))
     return set_err(GSKNN_EINVAL, "%s: null array", fn);
-  const bool dA = is_device_ptr(XA), dA2 = is_device_ptr(XA2), dam = is_device_ptr(amap);
-  const bool dB = is_device_ptr(XB), dB2 = is_device_ptr(XB2), dbm = is_device_ptr(bmap);
+  const bool dA = is_device_ptr(XA), dam = is_device_ptr(amap);
+  const bool dB = is_device_ptr(XB), dbm = is_device_ptr(bmap);
+  const bool dA2 = !norms || is_device_ptr(XA2), dB2 = !norms || is_device_ptr(XB2);
   int rc;
   int maxa = -1, maxb = -1;
   if (!dA || !dA2)
     if ((rc = get_max_index(amap, m, dam, &maxa, fn))) return rc;
   if (!dB || !dB2)
     if ((rc = get_max_index(bmap, n, dbm, &maxb, fn))) return rc;
-  const bool same = (XA == XB) && (XA2 == XB2) && !dA && !dA2;
+  const bool same = (XA == XB) && (XA2 == XB2) && !dA && (!norms || !dA2);
   if (same) maxa = maxb = std::max(maxa, maxb);
   *out = Inputs<T>{XA, XA2, XB, XB2, amap, bmap};
   void *p;
   if (!dA) { if ((rc = upload(g_buf[0], XA, sizeof(T) * (size_t)(maxa + 1) * d, &p, s))) return rc; out->XA = (T *)p; }
   if (!dA2) { if ((rc = upload(g_buf[1], XA2, sizeof(T) * (size_t)(maxa + 1), &p, s))) return rc; out->XA2 = (T *)p; }
+  if (!norms) out->XA2 = out->XB2 = nullptr;
   if (!dam) { if ((rc = upload(g_buf[2], amap, sizeof(int) * (size_t)m, &p, s))) return rc; out->amap = (int *)p; }
   if (same) {
     out->XB = out->XA;
-    out->XB2 = out->XA2;
+    out->XB2 = norms ? out->XA2 : nullptr;
   } else {
     if (!dB) { if ((rc = upload(g_buf[3], XB, sizeof(T) * (size_t)(maxb + 1) * d, &p, s))) return rc; out->XB = (T *)p; }
     if (!dB2) { if ((rc = upload(g_buf[4], XB2, sizeof(T) * (size_t)(maxb + 1), &p, s))) return rc; out->XB2 = (T *)p; }
@@ -412,16 +473,30 @@ int finish_rows(T *D, int *I, const T *gD, const int *gI, bool dD, bool dI, size
   return GSKNN_OK;
 }
 
+// mode >= 0: squared-2-norm kNN in that mode; kModeL1: L1 kNN.
+constexpr int kModeL1 = -1;
+
 template <typename T>
 hipError_t launch_any(const KnnArgs<T> &a, int mode, hipStream_t s, void *work, size_t wb) {
+  if (mode == kModeL1) {
+    if constexpr (sizeof(T) == 8)
+      return launch_knn_l1_f64(a, s, work, wb);
+    else
+      return launch_knn_l1_f32(a, s, work, wb);
+  }
   if constexpr (sizeof(T) == 8)
     return launch_knn_f64(a, mode, s, work, wb);
   else
     return launch_knn_f32(a, mode, s, work, wb);
 }
 
+size_t any_work_bytes(int m, int n, int d, int k, int elem, int mode) {
+  return mode == kModeL1 ? knn_l1_work_bytes(m, n, d, k, elem)
+                         : knn_work_bytes(m, n, d, k, elem, mode);
+}
+
 // One staged kNN call: host or device pointers in, result in D/I (row
-// stride ldk) when the call returns.
+// stride ldk) when the call returns.  mode = kModeL1 selects the L1 norm.
 template <typename T>
 int knn_host(const char *fn, int m, int n, int d, int k, const T *XA, const T *XA2,
              const int *amap, const T *XB, const T *XB2, const int *bmap, T *D, int *I,
@@ -434,12 +509,13 @@ int knn_host(const char *fn, int m, int n, int d, int k, const T *XA, const T *X
   int rc;
   if ((rc = begin_call(&s))) return rc;
   Inputs<T> in;
-  if ((rc = stage_inputs(fn, m, n, d, XA, XA2, amap, XB, XB2, bmap, s, &in))) return rc;
+  if ((rc = stage_inputs(fn, m, n, d, XA, XA2, amap, XB, XB2, bmap, s, &in, mode != kModeL1)))
+    return rc;
   T *gD;
   int *gI;
   bool dD, dI;
   if ((rc = stage_rows(D, I, n, ldk, s, &gD, &gI, &dD, &dI))) return rc;
-  const size_t wb = knn_work_bytes(m, n, d, k, (int)sizeof(T), mode);
+  const size_t wb = any_work_bytes(m, n, d, k, (int)sizeof(T), mode);
   void *work = nullptr;
   if (wb && (rc = ensure(g_buf[8], wb, &work))) return rc;
   KnnArgs<T> a{};
@@ -583,6 +659,39 @@ extern "C" void dgsknn(int m, int n, int k, int r, double *XA, double *XA2, int 
   } else {
     dgsknn_var1(n, m, k, r, XB, XB2, beta, XA, XA2, alpha, heap->D, heap->I);
   }
+}
+
+// L1 (KNN_1NORM) on heap rows of stride r, the dgsknn_var1 layout the
+// reference's L1 kernels run in (frame/gsknn.c:930); the reference has no L1
+// kernel for the r > 512 header layout.
+template <typename T>
+static void knn1_heap(const char *fn, int m, int n, int k, int r, T *XA, int *alpha, T *XB,
+                      int *beta, T *D, int *I) {
+  g_err[0] = 0;
+  if (r > KNN_VAR_THRES) {
+    set_err(GSKNN_ELIMIT, "%s: r=%d > %d (the large-r path has no L1 form)", fn, r,
+            KNN_VAR_THRES);
+    return;
+  }
+  (void)knn_host<T>(fn, m, n, k, r, XA, nullptr, alpha, XB, nullptr, beta, D, I, r, kModeL1);
+}
+
+extern "C" void dgsknn_1norm(int m, int n, int k, int r, double *XA, int *alpha, double *XB,
+                             int *beta, heap_t *heap) {
+  if (!heap) {
+    set_err(GSKNN_EINVAL, "dgsknn_1norm: null heap");
+    return;
+  }
+  knn1_heap("dgsknn_1norm", m, n, k, r, XA, alpha, XB, beta, heap->D, heap->I);
+}
+
+extern "C" void sgsknn_1norm(int m, int n, int k, int r, float *XA, int *alpha, float *XB,
+                             int *beta, heap_t *heap) {
+  if (!heap) {
+    set_err(GSKNN_EINVAL, "sgsknn_1norm: null heap");
+    return;
+  }
+  knn1_heap("sgsknn_1norm", m, n, k, r, XA, alpha, XB, beta, heap->D_s, heap->I);
 }
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,19 @@ hipError_t launch_knn_f32(const KnnArgs<float> &a, int mode, hipStream_t s,
 // Workspace bytes the launch above needs (partial lists of split launches).
 size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
 
+// Merge of split partial lists into the heap rows (gsknn_kernels.hip);
+// supported when a workgroup's heaps fit in LDS.
+bool knn_merge_supported(int k, int elem_bytes);
+hipError_t launch_knn_merge(const KnnArgs<double> &a, hipStream_t s);
+hipError_t launch_knn_merge(const KnnArgs<float> &a, hipStream_t s);
+
+// L1-norm (KNN_1NORM) kNN (gsknn_l1.hip); XA2/XB2 are not read.
+size_t knn_l1_work_bytes(int m, int n, int d, int k, int elem_bytes);
+hipError_t launch_knn_l1_f64(const KnnArgs<double> &a, hipStream_t s, void *work,
+                             size_t work_bytes);
+hipError_t launch_knn_l1_f32(const KnnArgs<float> &a, hipStream_t s, void *work,
+                             size_t work_bytes);
+
 // dgssq2nrm: C[i + j*ldc] for candidate i < m, query j < n.
 hipError_t launch_sq2nrm_f64(int m, int n, int d, const double *XA, const double *XA2,
                              const int *amap, const double *XB, const double *XB2,

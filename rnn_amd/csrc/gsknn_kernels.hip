@@ -919,6 +919,12 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
   return part_bytes(p, n, k, elem_bytes);
 }
 
+bool knn_merge_supported(int k, int elem_bytes) { return merge_qpb(k, elem_bytes) > 0; }
+
+hipError_t launch_knn_merge(const KnnArgs<double> &a, hipStream_t s) { return launch_merge(a, s); }
+
+hipError_t launch_knn_merge(const KnnArgs<float> &a, hipStream_t s) { return launch_merge(a, s); }
+
 hipError_t launch_knn_f64(const KnnArgs<double> &a, int mode, hipStream_t s, void *work,
                           size_t work_bytes) {
   return launch_knn<double>(a, mode, s, work, work_bytes);

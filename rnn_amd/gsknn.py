@@ -93,6 +93,15 @@ def _declare(L):
         getattr(L, name).argtypes = [i32] * 4 + [vp] * 7
         getattr(L, name).restype = None
     L.dgssq2nrm.argtypes = [i32] * 3 + [vp] * 7 + [i32]
+    for name in ("dgsknn_1norm", "sgsknn_1norm"):
+        getattr(L, name).argtypes = [i32] * 4 + [vp] * 5
+        getattr(L, name).restype = None
+    for name in ("gsknn_dknn1_dev", "gsknn_sknn1_dev"):
+        getattr(L, name).argtypes = [i32] * 4 + [vp] * 6 + [i32, vp, vp, sz, vp]
+        getattr(L, name).restype = i32
+    for name in ("gsknn_dknn1_work_bytes", "gsknn_sknn1_work_bytes"):
+        getattr(L, name).argtypes = [i32] * 4
+        getattr(L, name).restype = sz
     L.heapCreate_d.argtypes = [i32, i32, ctypes.c_double]
     L.heapCreate_d.restype = ctypes.POINTER(HeapT)
     L.heapCreate_s.argtypes = [i32, i32, ctypes.c_float]
@@ -188,6 +197,30 @@ def sgsknn_heap(XA, XA2, amap, XB, XB2, bmap, D, I):
         _free_shell(h)
 
 
+def dgsknn_1norm_heap(XA, amap, XB, bmap, D, I):
+    """L1 kNN (KNN_1NORM) through dgsknn_1norm on (n, k) rows attached as a
+    heap_t: m = amap.size candidates XA[amap], n = bmap.size queries."""
+    n, k = D.shape
+    h = lib().heapAttach_d(n, k, _p(D), _p(I))
+    try:
+        lib().dgsknn_1norm(amap.size, bmap.size, XA.shape[1], k, _p(XA), _p(amap), _p(XB),
+                           _p(bmap), ctypes.cast(h, ctypes.c_void_p))
+        _raise_if_error()
+    finally:
+        _free_shell(h)
+
+
+def sgsknn_1norm_heap(XA, amap, XB, bmap, D, I):
+    n, k = D.shape
+    h = lib().heapAttach_s(n, k, _p(D), _p(I))
+    try:
+        lib().sgsknn_1norm(amap.size, bmap.size, XA.shape[1], k, _p(XA), _p(amap), _p(XB),
+                           _p(bmap), ctypes.cast(h, ctypes.c_void_p))
+        _raise_if_error()
+    finally:
+        _free_shell(h)
+
+
 def _free_shell(h):
     # heapFree_[sd] frees D/I too (frame/gsknn_heap.c:321-337); the arrays here
     # belong to numpy, so only the heap_t shell is released.
@@ -234,6 +267,23 @@ def sknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work
                               wb, _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
+
+
+def knn1_dev(XA, amap, XB, bmap, D, I, hrow=None, work=None, stream=None):
+    """L1 kNN (KNN_1NORM) on device tensors, FP64 or FP32 by XA's dtype."""
+    k = D.shape[1]
+    m, n, d = amap.numel(), bmap.numel(), XA.shape[1]
+    wb = work.numel() * work.element_size() if work is not None else 0
+    fn = lib().gsknn_dknn1_dev if XA.element_size() == 8 else lib().gsknn_sknn1_dev
+    rc = fn(m, n, d, k, _dp(XA), _dp(amap), _dp(XB), _dp(bmap), _dp(D), _dp(I), D.stride(0),
+            _dp(hrow), _dp(work), wb, _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+
+
+def work_bytes_l1(m, n, d, k, elem_bytes=8) -> int:
+    fn = lib().gsknn_dknn1_work_bytes if elem_bytes == 8 else lib().gsknn_sknn1_work_bytes
+    return int(fn(m, n, d, k))
 
 
 def work_bytes_d(m, n, d, k, mode=MODE_AUTO) -> int:

@@ -110,3 +110,23 @@ def test_work_bytes_planning_is_host_only():
     assert gsknn.work_bytes_d(2_000_000, 10, 128, 32) > 0
     assert gsknn.work_bytes_d(65536, 65536, 64, 32) == 0
     assert gsknn.work_bytes_d(0, 10, 8, 4) == 0
+
+
+def test_l1_large_r_rejected_before_any_gpu_work():
+    # The reference's large-r header layout has no L1 form.
+    X = np.zeros((4, 3))
+    idx = np.arange(4, dtype=np.int32)
+    D = np.zeros((4, 600))
+    I = np.zeros((4, 600), dtype=np.int32)
+    try:
+        gsknn.dgsknn_1norm_heap(X, idx, X, idx, D, I)
+    except gsknn.GsknnError as e:
+        assert "r=600" in str(e)
+    else:
+        raise AssertionError("dgsknn_1norm accepted r = 600")
+
+
+def test_l1_work_bytes_planning_is_host_only():
+    assert gsknn.work_bytes_l1(2_000_000, 10, 64, 32, 8) > 0
+    assert gsknn.work_bytes_l1(2_000_000, 10, 100, 32, 4) > 0
+    assert gsknn.work_bytes_l1(0, 10, 8, 4, 8) == 0
