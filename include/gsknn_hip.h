@@ -96,8 +96,10 @@ int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
  * every iteration.  For each item the queries perm[start + q0 .. + qpb) are
  * matched against the candidates perm[start .. start + size) (gsknn with
  * amap = bmap = the leaf), merging into heap row = point id (stride ldk).
- * qpb in {16, 32, 64}; ct (candidates per LDS tile) a multiple of 4; grid =
- * persistent workgroups.  Distances use the reference oracle's arithmetic. */
+ * qpb in {16, 32, 64}; ct (candidates per LDS tile) a multiple of 4, unused
+ * for d in {8, 16, 32, 64}, where candidates reach the ALUs through scalar
+ * loads and LDS holds only the heaps; grid = persistent workgroups.
+ * Distances use the reference oracle's arithmetic. */
 size_t gsknn_sweep_lds_bytes(int elem_bytes, int d, int k, int qpb, int ct);
 int gsknn_sweep_leaves_sdev(int d, int k, const float *X, const float *X2,
                             const int *perm, const int *items,

@@ -28,6 +28,32 @@ __device__ __forceinline__ double xadd(double a, double b) { return __dadd_rn(a,
 __device__ __forceinline__ float xmul(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ float xadd(float a, float b) { return __fadd_rn(a, b); }
 
+__device__ __forceinline__ double xsub(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ float xsub(float a, float b) { return __fsub_rn(a, b); }
+
+// Constant-address-space view of a global array: uniform reads through it
+// become scalar loads (SGPR operands shared by the whole wave).  Only arrays
+// a kernel never writes are viewed so.
+template <typename T>
+using sptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ sptr<T> scalar_view(const T *p) {
+  return (sptr<T>)p;
+}
+
+// The same pointer, which the compiler must treat as computed from the TC
+// values: loads through it cannot be issued before every value exists.  The
+// asm statements are empty.  Used to keep one chunk's scalar loads from
+// being hoisted above the previous chunk's arithmetic (hoisting them all
+// would need more SGPRs than a wave has).
+template <typename T, int TC>
+__device__ __forceinline__ sptr<T> after(sptr<T> p, const T (&v)[TC]) {
+#pragma unroll
+  for (int u = 0; u < TC; ++u) asm volatile("" : "+s"(p) : "v"(v[u]));
+  return p;
+}
+
 // Sift (xd, xi) down from slot s of the heap of query q (k entries, LDS,
 // stride QS between entries).
 template <typename T>
@@ -259,6 +285,116 @@ __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
     }
   }
   if (valid) heap_store(a, hd, hi, QPB, q, j, split);
+}
+
+// ---------------------------------------------------------------------------
+// Exact block routine, scalar-candidate form (d == D in {8, 16, 32, 64}).
+// Same work, arithmetic and result as exact_block; the query row sits in
+// registers and candidates are walked by the wave in groups of kScalarTC,
+// their ids, coordinates and norms read with scalar loads and used as SGPR
+// operands, so each pair-dimension is one v_mul plus one v_add and
+// candidates cost no LDS or vector-memory traffic.  LDS holds only the heaps
+// (ExactScalarLayout).
+// ---------------------------------------------------------------------------
+constexpr int kScalarTC = 4;
+
+__host__ __device__ constexpr bool scalar_dim(int d) {
+  return d == 8 || d == 16 || d == 32 || d == 64;
+}
+
+template <typename T>
+struct ExactScalarLayout {
+  size_t hd, hi, bytes;
+  __host__ __device__ ExactScalarLayout(int k, int QPB) {
+    hd = 0;
+    hi = al16(sizeof(T) * (size_t)k * QPB);
+    bytes = al16(hi + sizeof(int) * (size_t)k * QPB);
+  }
+};
+
+template <typename T, int QPB, int D, bool FULL>
+__device__ __forceinline__ void exact_group_s(sptr<int> am, sptr<T> xs, sptr<T> x2, int c0,
+                                              int cend, const T (&qreg)[D], T qn, bool valid,
+                                              T *hd, int *hi, int q, int k, T &rd, int &ri) {
+  constexpr int TC = kScalarTC;
+  constexpr int P0 = 64 / (int)sizeof(T);  // one s_load_dwordx16 per candidate
+  constexpr int P = P0 < D ? P0 : D;
+  int id[TC];
+#pragma unroll
+  for (int u = 0; u < TC; ++u) id[u] = am[FULL ? c0 + u : min(c0 + u, cend - 1)];
+  T s[TC];
+#pragma unroll
+  for (int u = 0; u < TC; ++u) s[u] = T(0);
+#pragma unroll
+  for (int p0 = 0; p0 < D; p0 += P) {
+    const sptr<T> base = p0 == 0 ? xs : after<T, TC>(xs, s);
+    T cv[TC][P];
+#pragma unroll
+    for (int u = 0; u < TC; ++u)
+#pragma unroll
+      for (int i = 0; i < P; ++i) cv[u][i] = base[(size_t)id[u] * D + p0 + i];
+    // Cs += As[i,p] * Bs[j,p], in p order, product and sum rounded separately
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+#pragma unroll
+      for (int u = 0; u < TC; ++u) s[u] = xadd(s[u], xmul(cv[u][i], qreg[p0 + i]));
+  }
+  if (!valid) return;
+  const int cnt = FULL ? TC : cend - c0;
+#pragma unroll
+  for (int u = 0; u < TC; ++u) {
+    if (u >= cnt) continue;
+    // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
+    const T dist = xadd(xadd(xmul(s[u], T(-2)), x2[id[u]]), qn);
+    if (cless(dist, id[u], rd, ri)) {
+      sift_down(hd, hi, QPB, q, 0, k, dist, id[u]);
+      rd = hd[q];
+      ri = hi[q];
+    }
+  }
+}
+
+template <typename T, int QPB, int D>
+__device__ void exact_block_s(const KnnArgs<T> &a, int qblock, int split, unsigned char *smem) {
+  const int k = a.k;
+  const ExactScalarLayout<T> L(k, QPB);
+  T *hd = (T *)(smem + L.hd);
+  int *hi = (int *)(smem + L.hi);
+  const int q = threadIdx.x;
+  const int j = qblock * QPB + q;
+  const bool valid = j < a.n;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+
+  T qreg[D];
+  T qn = T(0), rd = T(0);
+  int ri = 0;
+  {
+    const T *xq = a.XB + (size_t)(valid ? a.bmap[j] : 0) * D;
+#pragma unroll
+    for (int p = 0; p < D; ++p) qreg[p] = valid ? xq[p] : T(0);
+  }
+  if (valid) {
+    qn = a.XB2[a.bmap[j]];
+    heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
+    rd = hd[q];
+    ri = hi[q];
+  }
+  const sptr<int> am = scalar_view(a.amap);
+  const sptr<T> xs = scalar_view(a.XA);
+  const sptr<T> x2 = scalar_view(a.XA2);
+  int c0 = cbeg;
+  for (; c0 + kScalarTC <= cend; c0 += kScalarTC)
+    exact_group_s<T, QPB, D, true>(am, xs, x2, c0, cend, qreg, qn, valid, hd, hi, q, k, rd, ri);
+  if (c0 < cend)
+    exact_group_s<T, QPB, D, false>(am, xs, x2, c0, cend, qreg, qn, valid, hd, hi, q, k, rd, ri);
+  if (valid) heap_store(a, hd, hi, QPB, q, j, split);
+}
+
+template <typename T, int QPB, int D>
+__global__ __launch_bounds__(QPB) void knn_exact_s_kernel(KnnArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  exact_block_s<T, QPB, D>(a, blockIdx.x, blockIdx.y, smem);
 }
 
 // Query rows in registers when d <= kExactDQ.

@@ -616,7 +616,7 @@ constexpr int kTargetBlocks = 512;      // 2 workgroups per CU on 256 CUs
 struct Plan {
   int kind = 0;  // 0 exact, 1 mfma
   int qpb = 64;  // queries per workgroup
-  int ct = 0;    // exact: candidates per tile
+  int ct = 0;    // exact: candidates per LDS tile (0: scalar-candidate form)
   int ks = 0, qb = 0, w = 0, nb = 1;  // mfma
   size_t lds = 0;
   int gx = 0, nsplit = 1, cps = 0;
@@ -754,27 +754,40 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
               k, ks, p.qb, p.w, p.nb, p.lds, best);
   }
   if (p.kind == 0) {
-    p.ct = exact_ct(d, elem);
+    // d in {8, 16, 32, 64}: the scalar-candidate form (heaps only in LDS).
+    const bool scalar = scalar_dim(d);
+    p.ct = scalar ? 0 : exact_ct(d, elem);
     p.qpb = 0;
     for (int qpb : {64, 32, 16}) {
       const bool qreg = d <= kExactDQ;
-      const size_t b = f64 ? ExactLayout<double>(k, qpb, d, p.ct, qreg).bytes
-                           : ExactLayout<float>(k, qpb, d, p.ct, qreg).bytes;
+      size_t b;
+      if (scalar)
+        b = f64 ? ExactScalarLayout<double>(k, qpb).bytes : ExactScalarLayout<float>(k, qpb).bytes;
+      else
+        b = f64 ? ExactLayout<double>(k, qpb, d, p.ct, qreg).bytes
+                : ExactLayout<float>(k, qpb, d, p.ct, qreg).bytes;
       if (b <= kLdsMax) {
         p.qpb = qpb;
         p.lds = b;
         break;
       }
     }
-    tile_c = p.ct;
+    tile_c = scalar ? 4 * kScalarTC : p.ct;
   }
   if (p.qpb == 0) return p;
   p.gx = (n + p.qpb - 1) / p.qpb;
   // Split the candidates when the query blocks alone cannot fill the chip;
-  // each split keeps at least 8 tiles of work.
+  // each split keeps at least 8 tiles of work.  The scalar-candidate form
+  // waits on scalar loads and wants every resident wave LDS allows, over
+  // two rounds of the chip.
+  int target = kTargetBlocks;
+  if (p.kind == 0 && p.ct == 0) {
+    const int per_cu = (int)std::min<size_t>(16, kLdsMax / p.lds);
+    target = 2 * 256 * per_cu;
+  }
   int ns = 1;
-  if (p.gx < kTargetBlocks && merge_qpb(k, elem) > 0) {
-    ns = (kTargetBlocks + p.gx - 1) / p.gx;
+  if (p.gx < target && merge_qpb(k, elem) > 0) {
+    ns = (target + p.gx - 1) / p.gx;
     const int maxs = m / (8 * tile_c);
     if (ns > maxs) ns = maxs;
     if (ns < 1) ns = 1;
@@ -825,8 +838,20 @@ template <typename T>
 hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   const dim3 grid(p.gx, p.nsplit);
   hipError_t e = hipSuccess;
+#define GSKNN_EXACT_S_CASE(QPB_, D_)                                          \
+  if (p.qpb == QPB_ && a.d == D_) {                                           \
+    auto kern = knn_exact_s_kernel<T, QPB_, D_>;                              \
+    if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
+    hipLaunchKernelGGL(kern, grid, dim3(QPB_), p.lds, s, a);                  \
+    return hipGetLastError();                                                 \
+  }
 #define GSKNN_EXACT_CASE(QPB_)                                                \
-  if (p.qpb == QPB_) {                                                        \
+  if (p.ct == 0) {                                                            \
+    GSKNN_EXACT_S_CASE(QPB_, 8)                                               \
+    GSKNN_EXACT_S_CASE(QPB_, 16)                                              \
+    GSKNN_EXACT_S_CASE(QPB_, 32)                                              \
+    GSKNN_EXACT_S_CASE(QPB_, 64)                                              \
+  } else if (p.qpb == QPB_) {                                                 \
     auto kern = a.d <= kExactDQ ? knn_exact_kernel<T, QPB_, kExactDQ>         \
                                 : knn_exact_kernel<T, QPB_, 0>;               \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
@@ -837,6 +862,7 @@ hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   GSKNN_EXACT_CASE(32)
   GSKNN_EXACT_CASE(16)
 #undef GSKNN_EXACT_CASE
+#undef GSKNN_EXACT_S_CASE
   return hipErrorNotSupported;
 }
 

@@ -25,29 +25,6 @@
 namespace gsknn {
 namespace {
 
-// Constant-address-space view of a global array: uniform reads through it
-// become scalar loads.  Only arrays the kernel never writes are viewed so.
-template <typename T>
-using sptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ sptr<T> scalar_view(const T *p) {
-  return (sptr<T>)p;
-}
-
-// The same pointer, which the compiler must treat as computed from the TC
-// sums: loads through it cannot be issued before every sum of the previous
-// chunk exists.  The asm statements are empty.
-template <typename T, int TC>
-__device__ __forceinline__ sptr<T> after(sptr<T> p, const T (&v)[TC]) {
-#pragma unroll
-  for (int u = 0; u < TC; ++u) asm volatile("" : "+s"(p) : "v"(v[u]));
-  return p;
-}
-
-__device__ __forceinline__ double xsub(double a, double b) { return __dsub_rn(a, b); }
-__device__ __forceinline__ float xsub(float a, float b) { return __fsub_rn(a, b); }
-
 constexpr int kL1Block = 256;  // DKNN_KC: dimensions per summation block
 constexpr int kL1TC = 4;       // candidates per group
 

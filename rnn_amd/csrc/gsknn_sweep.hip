@@ -11,14 +11,17 @@
 // captured once in a hipGraph serves every iteration.  Each workgroup walks
 // the list with stride gridDim.x (items sorted by leaf size, largest first,
 // balance the persistent grid).  Within an item the exact block routine runs
-// one thread per query with the oracle's arithmetic, so distances are
-// bit-identical to the oracle's.  Leaves of one iteration are disjoint, so no
+// one thread per query with the oracle's arithmetic (the scalar-candidate
+// form for d in {8, 16, 32, 64}), so distances are bit-identical to the
+// oracle's.  Leaves of one iteration are disjoint, so no
 // heap row is touched by two items of the same launch.
 #include "gsknn_device.h"
 
 namespace gsknn {
 
-template <typename T, int QPB, int DQ>
+// D > 0: d == D in {8, 16, 32, 64}, the scalar-candidate block routine
+// (CT unused); D == 0: the LDS-tile routine with DQ.
+template <typename T, int QPB, int DQ, int D>
 __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, const int4 *items,
                                                              const int *nitems, int CT) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -33,7 +36,10 @@ __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, co
     a.hrow = base.amap + w.x;
     a.nsplit = 1;
     a.cps = w.y;
-    exact_block<T, QPB, DQ>(a, CT, w.z / QPB, 0, smem);
+    if constexpr (D > 0)
+      exact_block_s<T, QPB, D>(a, w.z / QPB, 0, smem);
+    else
+      exact_block<T, QPB, DQ>(a, CT, w.z / QPB, 0, smem);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -86,6 +92,7 @@ hipError_t launch_tree_level_keys(long long total, int nnodes, int d, int id_bit
 
 template <typename T>
 size_t leaf_sweep_lds(int d, int k, int qpb, int ct) {
+  if (scalar_dim(d)) return ExactScalarLayout<T>(k, qpb).bytes;
   return ExactLayout<T>(k, qpb, d, ct, d <= kExactDQ).bytes;
 }
 
@@ -104,8 +111,12 @@ hipError_t launch_leaf_sweep_t(int d, int k, const T *X, const T *X2, const int 
   hipError_t e = hipSuccess;
 #define GSKNN_LEAF_CASE(QPB_)                                                             \
   if (qpb == QPB_) {                                                                      \
-    auto kern = d <= kExactDQ ? knn_leaf_sweep_kernel<T, QPB_, kExactDQ>                  \
-                              : knn_leaf_sweep_kernel<T, QPB_, 0>;                        \
+    auto kern = d == 8    ? knn_leaf_sweep_kernel<T, QPB_, 0, 8>                          \
+                : d == 16 ? knn_leaf_sweep_kernel<T, QPB_, 0, 16>                         \
+                : d == 32 ? knn_leaf_sweep_kernel<T, QPB_, 0, 32>                         \
+                : d == 64 ? knn_leaf_sweep_kernel<T, QPB_, 0, 64>                         \
+                : d <= kExactDQ ? knn_leaf_sweep_kernel<T, QPB_, kExactDQ, 0>             \
+                                : knn_leaf_sweep_kernel<T, QPB_, 0, 0>;                   \
     e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             (int)lds);                                                    \
     if (e != hipSuccess) return e;                                                        \

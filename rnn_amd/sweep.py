@@ -32,7 +32,10 @@ from . import gsknn
 
 QPB_CHOICES = (64, 32)
 CT_CHOICES = (16, 32, 64)
-GRID_CHOICES = (512, 1024, 2048, 4096, 8192)
+GRID_CHOICES = (512, 1024, 2048, 4096, 8192, 16384)
+# Dimensions with the scalar-candidate block routine: no LDS candidate tile,
+# so the tile size is not a tuning knob there.
+SCALAR_DIMS = (8, 16, 32, 64)
 
 
 def _orderable(f):
@@ -164,7 +167,7 @@ def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None):
     for qpb in QPB_CHOICES:
         items = torch.from_numpy(make_items(leaves, qpb)).to(dev)
         nitems = torch.tensor([items.shape[0]], dtype=torch.int32, device=dev)
-        for ct in CT_CHOICES:
+        for ct in (CT_CHOICES[:1] if d in SCALAR_DIMS else CT_CHOICES):
             if int(gsknn.lib().gsknn_sweep_lds_bytes(elem, d, k, qpb, ct)) > 160 * 1024:
                 continue
             for grid in GRID_CHOICES:
