@@ -342,6 +342,55 @@ def serving(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+def c1_small(torch, dev, reps=200):
+    """Config 1: N = 8192, d = 16, U[0,1), m = n = 2048, k = 16, +inf heaps;
+    the call pair (pi[0:2048] then the disjoint candidates pi[2048:4096] merged
+    into the same rows).  Latency-bound (everything fits in cache): reports us
+    per call, pairs/s and the compulsory bytes per call
+    (m+n)(8d+8+4) + 2nk(8+4) over the call time against the 8 TB/s HBM peak."""
+    from rnn_amd import datasets as ds
+    from rnn_amd import gsknn
+
+    p = ds.config1()
+    X = torch.from_numpy(p.XA).to(dev)
+    X2 = torch.from_numpy(p.XA2).to(dev)
+    am1 = torch.from_numpy(p.amap).to(dev)
+    am2 = torch.from_numpy(p.extra_amaps[0]).to(dev)
+    bm = torch.from_numpy(p.bmap).to(dev)
+    D = torch.empty((p.n, p.k), dtype=torch.float64, device=dev)
+    I = torch.empty((p.n, p.k), dtype=torch.int32, device=dev)
+    wb = gsknn.work_bytes_d(p.m, p.n, p.d, p.k)
+    work = torch.empty(max(16, wb), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def pair():
+        gsknn.dknn_dev(X, X2, am1, X, X2, bm, D, I, work=work, stream=stream)
+        gsknn.dknn_dev(X, X2, am2, X, X2, bm, D, I, work=work, stream=stream)
+
+    for _ in range(5):
+        D.fill_(float("inf"))
+        I.fill_(-1)
+        pair()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    total = 0.0
+    for _ in range(reps):
+        D.fill_(float("inf"))
+        I.fill_(-1)
+        e0.record(stream)
+        pair()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        total += e0.elapsed_time(e1)
+    us = total * 1e3 / (2 * reps)
+    pairs = float(p.m) * p.n
+    B = (p.m + p.n) * (8 * p.d + 8 + 4) + 2 * p.n * p.k * (8 + 4)
+    return {"us_per_call": us, "pairs_per_s": pairs / (us * 1e-6), "bytes_per_call": B,
+            "gb_per_s": B / (us * 1e-6) / 1e9, "frac_hbm_peak": B / (us * 1e-6) / HBM_PEAK,
+            "calls_timed": 2 * reps}
+
+
 FP64_VALU_OPS_PEAK = 39.3e12  # 78.6 TFLOP/s FP64 vector counts an FMA as 2
 
 
@@ -405,7 +454,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "l1", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "l1", "copy"])
     ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
     ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
@@ -439,6 +488,9 @@ def main():
         return
     if args.workload == "l1":
         l1_norm(torch, dev, args)
+        return
+    if args.workload == "c1":
+        print(json.dumps({"workload": "c1", **c1_small(torch, dev)}), flush=True)
         return
     t0 = time.perf_counter()
     p = ds.config2()
@@ -561,6 +613,7 @@ def main():
         "pcie_inclusive": {"ms": host_ms, "pairs_per_s": pairs / (host_ms * 1e-3),
                            "rows_equal_device": host_equal},
         "rows_ascending": {"ms": sort_ms, "rows": p.n, "k": p.k, "order_ok": sort_ok},
+        "c1": c1_small(torch, dev),
     }
     if not args.no_cpu:
         log("cpu baseline ...")

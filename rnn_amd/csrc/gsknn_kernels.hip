@@ -485,20 +485,130 @@ __global__ __launch_bounds__(QPB) void knn_merge_kernel(KnnArgs<T> a) {
   heap_load(a, hd, hi, QPB, q, j, false);
   T rd = hd[q];
   int ri = hi[q];
-  for (int s = 0; s < a.nsplit; ++s) {
-    for (int t = 0; t < k; ++t) {
-      const size_t o = ((size_t)s * k + t) * a.n + j;
-      const T xd = a.partD[o];
-      const int xi = a.partI[o];
-      if (!cless(xd, xi, rd, ri)) break;
-      sift_down(hd, hi, QPB, q, 0, k, xd, xi);
-      rd = hd[q];
-      ri = hi[q];
+  // The head entries of up to kHeads lists are loaded together (independent
+  // loads in flight at once); a list is read further only while it beats the
+  // root, which few lists do.
+  constexpr int kHeads = 8;
+  for (int s0 = 0; s0 < a.nsplit; s0 += kHeads) {
+    T h_d[kHeads];
+    int h_i[kHeads];
+#pragma unroll
+    for (int u = 0; u < kHeads; ++u) {
+      const size_t o = ((size_t)min(s0 + u, a.nsplit - 1) * k) * a.n + j;
+      h_d[u] = a.partD[o];
+      h_i[u] = a.partI[o];
+    }
+#pragma unroll
+    for (int u = 0; u < kHeads; ++u) {
+      if (s0 + u >= a.nsplit) break;
+      T xd = h_d[u];
+      int xi = h_i[u];
+      for (int t = 0; cless(xd, xi, rd, ri);) {
+        sift_down(hd, hi, QPB, q, 0, k, xd, xi);
+        rd = hd[q];
+        ri = hi[q];
+        if (++t >= k) break;
+        const size_t o = ((size_t)(s0 + u) * k + t) * a.n + j;
+        xd = a.partD[o];
+        xi = a.partI[o];
+      }
     }
   }
   KnnArgs<T> b = a;
   b.nsplit = 1;
   heap_store(b, hd, hi, QPB, q, j, 0);
+}
+
+// Merge for calls with few queries: one wave per query.  The 64 lanes load
+// the resident row and every partial list at once (up to 64 * R entries), drop
+// list entries that do not beat the resident row's maximum (the split
+// launches' sentinels among them), and extract the k smallest in canonical
+// order by k rounds of a wave-wide arg-min; identical (distance, index) pairs
+// are distinct entries, broken by slot.  Same result as knn_merge_kernel.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void knn_merge_wave_kernel(KnnArgs<T> a) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= a.n) return;  // whole wave
+  const int k = a.k;
+  const size_t row = (size_t)heap_row(a, j) * a.ldk;
+  const int E = k * (1 + a.nsplit);
+  T ed[R];
+  int ei[R];
+  // Entry e < k: resident slot e; e >= k: list (e - k) / k, position (e - k) % k.
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = r * 64 + lane;
+    ed[r] = (T)__builtin_inf();
+    ei[r] = INT_MAX;
+    if (e < k) {
+      ed[r] = a.D[row + e];
+      ei[r] = a.I[row + e];
+    } else if (e < E) {
+      const int s = (e - k) / k, t = (e - k) % k;
+      const size_t o = ((size_t)s * k + t) * a.n + j;
+      ed[r] = a.partD[o];
+      ei[r] = a.partI[o];
+    }
+  }
+  // Resident maximum (md, mi): list entries must beat it strictly.
+  T md = (T)-__builtin_inf();
+  int mi = INT_MIN;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (r * 64 + lane < k && cless(md, mi, ed[r], ei[r])) {
+      md = ed[r];
+      mi = ei[r];
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    const T od = __shfl_xor(md, off);
+    const int oi = __shfl_xor(mi, off);
+    if (cless(md, mi, od, oi)) {
+      md = od;
+      mi = oi;
+    }
+  }
+  bool live[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = r * 64 + lane;
+    live[r] = e < k || (e < E && cless(ed[r], ei[r], md, mi));
+  }
+  // k rounds: each lane offers its smallest live entry; the wave minimum by
+  // (distance, index, lane) is written and retired.
+  for (int t = 0; t < k; ++t) {
+    T ld = (T)__builtin_inf();
+    int li = INT_MAX, lr = -1;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (live[r] && (lr < 0 || cless(ed[r], ei[r], ld, li))) {
+        ld = ed[r];
+        li = ei[r];
+        lr = r;
+      }
+    T bd = ld;
+    int bi = li, bl = lr >= 0 ? lane : 64;
+    for (int off = 32; off > 0; off >>= 1) {
+      const T od = __shfl_xor(bd, off);
+      const int oi = __shfl_xor(bi, off);
+      const int ol = __shfl_xor(bl, off);
+      if (ol < 64 && (bl == 64 || cless(od, oi, bd, bi) || (od == bd && oi == bi && ol < bl))) {
+        bd = od;
+        bi = oi;
+        bl = ol;
+      }
+    }
+    if (lane == bl) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r == lr) live[r] = false;
+    }
+    // Rows are written descending, as heap_store does.
+    if (lane == 0) {
+      a.D[row + (k - 1 - t)] = bd;
+      a.I[row + (k - 1 - t)] = bi;
+    }
+  }
 }
 
 // Rows to ascending (distance, index) order, in place: each 64-lane
@@ -772,7 +882,8 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
         break;
       }
     }
-    tile_c = scalar ? 4 * kScalarTC : p.ct;
+    // Scalar form: a split keeps at least 8 x 8 = 64 candidates.
+    tile_c = scalar ? 2 * kScalarTC : p.ct;
   }
   if (p.qpb == 0) return p;
   p.gx = (n + p.qpb - 1) / p.qpb;
@@ -866,8 +977,25 @@ hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   return hipErrorNotSupported;
 }
 
+// Calls with fewer queries than this merge with one wave per query.
+constexpr int kWaveMergeMaxN = 16384;
+
 template <typename T>
 hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
+  const int E = a.k * (1 + a.nsplit);
+  if (a.n <= kWaveMergeMaxN && E <= 64 * 16) {
+    const dim3 grid((a.n + 3) / 4);
+#define GSKNN_WMERGE_CASE(R_)                                                 \
+    if (E <= 64 * R_) {                                                       \
+      hipLaunchKernelGGL((knn_merge_wave_kernel<T, R_>), grid, dim3(256), 0, s, a); \
+      return hipGetLastError();                                               \
+    }
+    GSKNN_WMERGE_CASE(2)
+    GSKNN_WMERGE_CASE(4)
+    GSKNN_WMERGE_CASE(8)
+    GSKNN_WMERGE_CASE(16)
+#undef GSKNN_WMERGE_CASE
+  }
   const int qpb = merge_qpb(a.k, (int)sizeof(T));
   const size_t lds = merge_lds(a.k, (int)sizeof(T), qpb);
   hipError_t e = hipSuccess;

@@ -462,3 +462,33 @@ def test_rows_ascending_after_knn_and_example():
     assert np.array_equal(tD.cpu().numpy().view(np.uint8), Dr.view(np.uint8))
     assert example.main([]) == 0
     assert example.main(["--double"]) == 0
+
+
+@pytest.mark.parametrize("dtype,d", [(np.float64, 16), (np.float32, 32), (np.float64, 64),
+                                     (np.float32, 8)])
+def test_exact_scalar_candidate_form(dtype, d):
+    """Exact mode at >= 2^26 pairs with d in {8, 16, 32, 64} takes the
+    scalar-candidate block routine (candidates as SGPR operands): 48 sampled
+    rows bitwise equal to the oracle, partial last group and candidate splits
+    included."""
+    import torch
+    print(f"[gpu] exact scalar form {np.dtype(dtype).name} d={d}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(300 + d))
+    N = 12000
+    X = g.standard_normal((N, d)).astype(dtype)
+    X2 = oracle.sqnorm(X)
+    amap = g.integers(0, N, 8195).astype(np.int32)
+    bmap = g.integers(0, N, 8192).astype(np.int32)
+    k = 20
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((8192, k), float("inf"), dtype=tX.dtype, device=dev)
+    I = torch.full((8192, k), -1, dtype=torch.int32, device=dev)
+    call = gsknn.dknn_dev if dtype == np.float64 else gsknn.sknn_dev
+    call(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2, torch.from_numpy(bmap).to(dev), D, I,
+         mode=gsknn.MODE_EXACT)
+    torch.cuda.synchronize()
+    rows = np.linspace(0, 8191, 48).astype(np.int64)
+    Dr, Ir = ds.empty_heap(rows.size, k, dtype)
+    oracle.gsknn(X, X2, amap, X, X2, bmap[rows].copy(), Dr, Ir, canonical=True)
+    assert_rows(D.cpu().numpy()[rows], I.cpu().numpy()[rows], Dr, Ir)
