@@ -288,6 +288,90 @@ __device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
 }
 
 // ---------------------------------------------------------------------------
+// Per-lane survivor queue for thread-per-query heaps.  A candidate that beats
+// the lane's root as of the last flush waits in the queue; when some lane's
+// queue would overflow, every lane applies its queue (and the group at hand)
+// to its own heap at once, so heap updates run side by side instead of one
+// divergent lane at a time.  A stale root only lets more candidates through;
+// each is tested exactly against the live root when applied.  The heap is
+// ordered canonically, so its final contents do not depend on the order of
+// application.
+// ---------------------------------------------------------------------------
+template <typename T, int QR>
+struct LaneQueue {
+  T d[QR];
+  int i[QR];
+  int n;
+};
+
+template <typename T, int QR>
+__device__ __forceinline__ void queue_init(LaneQueue<T, QR> &Q) {
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+    Q.d[r] = T(0);
+    Q.i[r] = 0;
+  }
+  Q.n = 0;
+}
+
+template <typename T, int QR, int TC>
+__device__ __forceinline__ void queue_apply(LaneQueue<T, QR> &Q, const T (&dist)[TC],
+                                            const int (&id)[TC], unsigned pass, T *hd, int *hi,
+                                            int QS, int q, int k, T &rd, int &ri) {
+#pragma unroll
+  for (int r = 0; r < QR; ++r)
+    if (r < Q.n && cless(Q.d[r], Q.i[r], hd[q], hi[q])) sift_down(hd, hi, QS, q, 0, k, Q.d[r], Q.i[r]);
+#pragma unroll
+  for (int u = 0; u < TC; ++u)
+    if (((pass >> u) & 1u) && cless(dist[u], id[u], hd[q], hi[q]))
+      sift_down(hd, hi, QS, q, 0, k, dist[u], id[u]);
+  Q.n = 0;
+  rd = hd[q];
+  ri = hi[q];
+}
+
+// Offer a group of TC distances (the first cnt valid) of lane q.  Every lane
+// of the wave must call it (the overflow test is wave-wide).
+template <typename T, int QR, int TC>
+__device__ __forceinline__ void queue_offer(LaneQueue<T, QR> &Q, const T (&dist)[TC],
+                                            const int (&id)[TC], int cnt, bool valid, T *hd,
+                                            int *hi, int QS, int q, int k, T &rd, int &ri) {
+  unsigned pass = 0u;
+#pragma unroll
+  for (int u = 0; u < TC; ++u)
+    if (valid && u < cnt && cless(dist[u], id[u], rd, ri)) pass |= 1u << u;
+  if (__any(Q.n + __builtin_popcount(pass) > QR)) {
+    if (valid) queue_apply<T, QR, TC>(Q, dist, id, pass, hd, hi, QS, q, k, rd, ri);
+  } else {
+#pragma unroll
+    for (int u = 0; u < TC; ++u) {
+      if ((pass >> u) & 1u) {
+#pragma unroll
+        for (int r = 0; r < QR; ++r)
+          if (Q.n == r) {
+            Q.d[r] = dist[u];
+            Q.i[r] = id[u];
+          }
+        ++Q.n;
+      }
+    }
+  }
+}
+
+// Apply what is left in the queues (every lane of the wave calls it).
+template <typename T, int QR>
+__device__ __forceinline__ void queue_drain(LaneQueue<T, QR> &Q, bool valid, T *hd, int *hi,
+                                            int QS, int q, int k, T &rd, int &ri) {
+  if (__any(Q.n > 0)) {
+    const T nd[1] = {T(0)};
+    const int ni[1] = {0};
+    if (valid) queue_apply<T, QR, 1>(Q, nd, ni, 0u, hd, hi, QS, q, k, rd, ri);
+  }
+}
+
+constexpr int kLaneQueue = 8;
+
+// ---------------------------------------------------------------------------
 // Exact block routine, scalar-candidate form (d == D in {8, 16, 32, 64}).
 // Same work, arithmetic and result as exact_block; the query row sits in
 // registers and candidates are walked by the wave in groups of kScalarTC,
@@ -315,7 +399,8 @@ struct ExactScalarLayout {
 template <typename T, int QPB, int D, bool FULL>
 __device__ __forceinline__ void exact_group_s(sptr<int> am, sptr<T> xs, sptr<T> x2, int c0,
                                               int cend, const T (&qreg)[D], T qn, bool valid,
-                                              T *hd, int *hi, int q, int k, T &rd, int &ri) {
+                                              LaneQueue<T, kLaneQueue> &Q, T *hd, int *hi, int q,
+                                              int k, T &rd, int &ri) {
   constexpr int TC = kScalarTC;
   constexpr int P0 = 64 / (int)sizeof(T);  // one s_load_dwordx16 per candidate
   constexpr int P = P0 < D ? P0 : D;
@@ -339,19 +424,12 @@ __device__ __forceinline__ void exact_group_s(sptr<int> am, sptr<T> xs, sptr<T> 
 #pragma unroll
       for (int u = 0; u < TC; ++u) s[u] = xadd(s[u], xmul(cv[u][i], qreg[p0 + i]));
   }
-  if (!valid) return;
-  const int cnt = FULL ? TC : cend - c0;
+  // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
+  T dist[TC];
 #pragma unroll
-  for (int u = 0; u < TC; ++u) {
-    if (u >= cnt) continue;
-    // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
-    const T dist = xadd(xadd(xmul(s[u], T(-2)), x2[id[u]]), qn);
-    if (cless(dist, id[u], rd, ri)) {
-      sift_down(hd, hi, QPB, q, 0, k, dist, id[u]);
-      rd = hd[q];
-      ri = hi[q];
-    }
-  }
+  for (int u = 0; u < TC; ++u) dist[u] = xadd(xadd(xmul(s[u], T(-2)), x2[id[u]]), qn);
+  queue_offer<T, kLaneQueue, TC>(Q, dist, id, FULL ? TC : cend - c0, valid, hd, hi, QPB, q, k,
+                                 rd, ri);
 }
 
 template <typename T, int QPB, int D>
@@ -383,11 +461,15 @@ __device__ void exact_block_s(const KnnArgs<T> &a, int qblock, int split, unsign
   const sptr<int> am = scalar_view(a.amap);
   const sptr<T> xs = scalar_view(a.XA);
   const sptr<T> x2 = scalar_view(a.XA2);
+  LaneQueue<T, kLaneQueue> Q;
+  queue_init(Q);
   int c0 = cbeg;
   for (; c0 + kScalarTC <= cend; c0 += kScalarTC)
-    exact_group_s<T, QPB, D, true>(am, xs, x2, c0, cend, qreg, qn, valid, hd, hi, q, k, rd, ri);
+    exact_group_s<T, QPB, D, true>(am, xs, x2, c0, cend, qreg, qn, valid, Q, hd, hi, q, k, rd, ri);
   if (c0 < cend)
-    exact_group_s<T, QPB, D, false>(am, xs, x2, c0, cend, qreg, qn, valid, hd, hi, q, k, rd, ri);
+    exact_group_s<T, QPB, D, false>(am, xs, x2, c0, cend, qreg, qn, valid, Q, hd, hi, q, k, rd,
+                                    ri);
+  queue_drain(Q, valid, hd, hi, QPB, q, k, rd, ri);
   if (valid) heap_store(a, hd, hi, QPB, q, j, split);
 }
 

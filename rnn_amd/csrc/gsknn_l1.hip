@@ -16,8 +16,9 @@
 // step one v_add_f64 with an |x| source modifier, and candidates cost no LDS
 // or vector-memory traffic.  The query row sits in registers for d = 8, 16,
 // 32, 64 (D = d) and in LDS, one column per thread, otherwise (D = 0).
-// Each thread's heap lives in LDS, as in the exact kernel; split launches
-// write ascending partial lists that knn_merge_kernel folds into the rows.
+// Each thread's heap lives in LDS, as in the exact kernel, fed through a
+// per-lane survivor queue; split launches write ascending partial lists that
+// the merge kernels fold into the rows.
 #include <limits.h>
 
 #include "gsknn_device.h"
@@ -64,25 +65,12 @@ __device__ __forceinline__ void chunk_sum(const T *qv, const T (&cv)[TC][P], T (
     for (int u = 0; u < TC; ++u) acc[u] = xadd(acc[u], fabs(xsub(qv[i], cv[u][i])));
 }
 
-// Offer the group's distances to this thread's heap (canonical order).
-template <typename T, int TC>
-__device__ __forceinline__ void offer(const T (&tot)[TC], const int (&id)[TC], int cnt, T *hd,
-                                      int *hi, int QS, int q, int k, T &rd, int &ri) {
-#pragma unroll
-  for (int u = 0; u < TC; ++u) {
-    if (u < cnt && cless(tot[u], id[u], rd, ri)) {
-      sift_down(hd, hi, QS, q, 0, k, tot[u], id[u]);
-      rd = hd[q];
-      ri = hi[q];
-    }
-  }
-}
-
 // d == D (8..64): the query row in registers, one summation block, no
 // partial chunks.
 template <typename T, int QPB, int D, int TC, bool FULL>
 __device__ __forceinline__ void group_reg(sptr<int> am, sptr<T> xs, int c0, int cend,
-                                          const T (&qreg)[D], bool valid, T *hd, int *hi, int q,
+                                          const T (&qreg)[D], bool valid,
+                                          LaneQueue<T, kLaneQueue> &Q, T *hd, int *hi, int q,
                                           int k, T &rd, int &ri) {
   constexpr int P = l1_chunk<T>() < D ? l1_chunk<T>() : D;
   int id[TC];
@@ -102,15 +90,16 @@ __device__ __forceinline__ void group_reg(sptr<int> am, sptr<T> xs, int c0, int 
       for (int i = 0; i < P; ++i) cv[u][i] = base[(size_t)id[u] * D + p0 + i];
     chunk_sum<T, TC, P>(qreg + p0, cv, tot);
   }
-  if (valid) offer<T, TC>(tot, id, FULL ? TC : cend - c0, hd, hi, QPB, q, k, rd, ri);
+  queue_offer<T, kLaneQueue, TC>(Q, tot, id, FULL ? TC : cend - c0, valid, hd, hi, QPB, q, k,
+                                 rd, ri);
 }
 
 // Any d: the query row in this thread's LDS column; blocks of kL1Block
 // dimensions, the partial chunk at the end of the row loaded element-wise.
 template <typename T, int QPB, int TC, bool FULL>
 __device__ __forceinline__ void group_lds(sptr<int> am, sptr<T> xs, int c0, int cend, int d,
-                                          const T *sq, bool valid, T *hd, int *hi, int q, int k,
-                                          T &rd, int &ri) {
+                                          const T *sq, bool valid, LaneQueue<T, kLaneQueue> &Q,
+                                          T *hd, int *hi, int q, int k, T &rd, int &ri) {
   constexpr int P = l1_chunk<T>();
   int id[TC];
   group_ids<TC, FULL>(am, c0, cend, id);
@@ -140,7 +129,8 @@ __device__ __forceinline__ void group_lds(sptr<int> am, sptr<T> xs, int c0, int 
 #pragma unroll
     for (int u = 0; u < TC; ++u) tot[u] = b0 == 0 ? blk[u] : xadd(tot[u], blk[u]);
   }
-  if (valid) offer<T, TC>(tot, id, FULL ? TC : cend - c0, hd, hi, QPB, q, k, rd, ri);
+  queue_offer<T, kLaneQueue, TC>(Q, tot, id, FULL ? TC : cend - c0, valid, hd, hi, QPB, q, k,
+                                 rd, ri);
 }
 
 // D > 0: d == D with the query row in registers; D == 0: any d.
@@ -183,19 +173,22 @@ __global__ __launch_bounds__(QPB) void knn_l1_kernel(KnnArgs<T> a) {
 
   const sptr<int> am = scalar_view(a.amap);
   const sptr<T> xs = scalar_view(a.XA);
+  LaneQueue<T, kLaneQueue> Q;
+  queue_init(Q);
   int c0 = cbeg;
   for (; c0 + TC <= cend; c0 += TC) {
     if constexpr (D > 0)
-      group_reg<T, QPB, D, TC, true>(am, xs, c0, cend, qreg, valid, hd, hi, q, k, rd, ri);
+      group_reg<T, QPB, D, TC, true>(am, xs, c0, cend, qreg, valid, Q, hd, hi, q, k, rd, ri);
     else
-      group_lds<T, QPB, TC, true>(am, xs, c0, cend, d, sq, valid, hd, hi, q, k, rd, ri);
+      group_lds<T, QPB, TC, true>(am, xs, c0, cend, d, sq, valid, Q, hd, hi, q, k, rd, ri);
   }
   if (c0 < cend) {
     if constexpr (D > 0)
-      group_reg<T, QPB, D, TC, false>(am, xs, c0, cend, qreg, valid, hd, hi, q, k, rd, ri);
+      group_reg<T, QPB, D, TC, false>(am, xs, c0, cend, qreg, valid, Q, hd, hi, q, k, rd, ri);
     else
-      group_lds<T, QPB, TC, false>(am, xs, c0, cend, d, sq, valid, hd, hi, q, k, rd, ri);
+      group_lds<T, QPB, TC, false>(am, xs, c0, cend, d, sq, valid, Q, hd, hi, q, k, rd, ri);
   }
+  queue_drain(Q, valid, hd, hi, QPB, q, k, rd, ri);
   if (valid) heap_store(a, hd, hi, QPB, q, j, split);
 }
 
