@@ -51,26 +51,48 @@ __global__ __launch_bounds__(QPB) void knn_leaf_sweep_kernel(KnnArgs<T> base, co
 // (projection, id), ties by id, so the median split is deterministic.  The
 // projection is sum_p x_p dir_p in p order, products and sums rounded
 // separately; its FP32 bits map to an unsigned order (negative values
-// reversed).
-__global__ void tree_level_keys_kernel(long long total, int nnodes, int d, int id_bits,
-                                       const float *X, const long long *perm,
-                                       const long long *starts, const long long *offs,
-                                       const float *dirs, long long *pos_out,
-                                       long long *key_out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  int lo = 0, hi = nnodes - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (offs[mid] <= i) lo = mid;
-    else hi = mid - 1;
+// reversed).  Blocks of 256 consecutive elements: one thread finds the node
+// of the block's first element by binary search and the others step forward
+// from it (a block spans few nodes); rows are read 16 bytes at a time when d
+// is a multiple of 4 and the tables are 16-byte aligned.
+__global__ __launch_bounds__(256) void tree_level_keys_kernel(
+    long long total, int nnodes, int d, int id_bits, const float *X, const long long *perm,
+    const long long *starts, const long long *offs, const float *dirs, long long *pos_out,
+    long long *key_out) {
+  __shared__ int base_node;
+  const long long i0 = (long long)blockIdx.x * blockDim.x;
+  const long long i = i0 + threadIdx.x;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = nnodes - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (offs[mid] <= i0) lo = mid;
+      else hi = mid - 1;
+    }
+    base_node = lo;
   }
+  __syncthreads();
+  if (i >= total) return;
+  int lo = base_node;
+  while (lo + 1 < nnodes && offs[lo + 1] <= i) ++lo;
   const long long pos = starts[lo] + (i - offs[lo]);
   const long long pt = perm[pos];
   const float *x = X + pt * d;
   const float *dv = dirs + (size_t)lo * d;
   float s = 0.0f;
-  for (int p = 0; p < d; ++p) s = __fadd_rn(s, __fmul_rn(x[p], dv[p]));
+  if ((d & 3) == 0 && (((uintptr_t)X | (uintptr_t)dirs) & 15) == 0) {
+    const float4 *x4 = (const float4 *)x;
+    const float4 *d4 = (const float4 *)dv;
+    for (int p = 0; p < d / 4; ++p) {
+      const float4 a = x4[p], w = d4[p];
+      s = __fadd_rn(s, __fmul_rn(a.x, w.x));
+      s = __fadd_rn(s, __fmul_rn(a.y, w.y));
+      s = __fadd_rn(s, __fmul_rn(a.z, w.z));
+      s = __fadd_rn(s, __fmul_rn(a.w, w.w));
+    }
+  } else {
+    for (int p = 0; p < d; ++p) s = __fadd_rn(s, __fmul_rn(x[p], dv[p]));
+  }
   int b = __float_as_int(s);
   if (b < 0) b ^= 0x7FFFFFFF;
   const long long ord = (long long)b + (1LL << 31);
