@@ -10,9 +10,11 @@ random candidates.  One step = restore the pre-filled heap rows (device copy)
 
 Prints one JSON line.  `roofline` prices the dominant kernel against the
 FP64 matrix peak using 2*d flops per pair and the kernel's average duration
-from HIP events on the launch stream.  `cpu_baseline` times the repository's
-C restatement of the reference oracle on a bounded sample of the same
-workload (all 65536 candidates against a slice of the queries) on the host.
+from HIP events on the launch stream.  `cpu_baseline` times the reference's
+own optimized dgsknn (compiled from its sources into oracle/_ref) on a
+bounded sample of the same workload (all 65536 candidates against a slice of
+the queries) on the host, with the repository's C restatement of the oracle
+beside it.
 """
 from __future__ import annotations
 
@@ -81,44 +83,61 @@ def _cpu_model():
 
 
 def cpu_baseline(p, seconds_per_run=2.0):
-    """Oracle (C restatement, OpenMP) on all candidates x a slice of queries.
-
-    The slice is sized so one run takes about `seconds_per_run`; the figure is
-    the median of 3 runs, at the host's thread share (at most 16) and again at
-    1 thread."""
+    """The reference's own optimized dgsknn (oracle/_ref/libgsknn_opt.so: its
+    frame/gsknn.c with the x86-64 AVX micro-kernels, compiled from its
+    sources; GSKNN_IC_NT threads) on all candidates x a slice of the queries,
+    when that library was built, else the repository's C restatement of the
+    oracle.  The slice is sized so one run takes about `seconds_per_run`; each
+    figure is the median of 3 runs, at the host's thread share (at most 16)
+    and at 1 thread; the restatement's figure is reported beside the
+    reference's."""
     import numpy as np
 
     from oracle import oracle
 
-    def run(nq, threads):
-        D = np.full((nq, p.k), np.inf)
+    have_ref = oracle.ref_opt_lib() is not None
+
+    def run(nq, threads, ref):
+        D = np.full((nq, p.k), np.finfo(np.float64).max if ref else np.inf)
         I = np.full((nq, p.k), -1, dtype=np.int32)
         bm = p.bmap[:nq].copy()
         t0 = time.perf_counter()
-        oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, D, I, threads=threads)
+        if ref:
+            oracle.ref_opt_gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, D, I, threads=threads)
+        else:
+            oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, bm, D, I, threads=threads)
         return time.perf_counter() - t0
 
-    def measure(threads):
+    def measure(threads, ref):
         nq = 16
         while True:
-            dt = run(nq, threads)
+            dt = run(nq, threads, ref)
             if dt > seconds_per_run / 2 or nq >= p.n:
                 break
             nq = min(p.n, int(nq * max(2.0, seconds_per_run / max(dt, 1e-3))))
-        times = sorted(run(nq, threads) for _ in range(3))
+        times = sorted(run(nq, threads, ref) for _ in range(3))
         return nq, times[1]
 
     nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(16, nproc or 1))
-    nq, dt = measure(threads)
-    nq1, dt1 = measure(1)
-    return {"value": p.m * nq / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{nq} queries x {p.m} candidates (d={p.d}, k={p.k}) of the same call, "
-                      f"median of 3 runs of {dt:.2f} s, oracle/gsknn_oracle.c -O3 -fopenmp",
-            "single_thread": {"value": p.m * nq1 / dt1, "unit": "pairs/s", "cores": 1,
-                              "sample": f"{nq1} queries x {p.m} candidates, median of 3 runs "
-                                        f"of {dt1:.2f} s"},
-            "nproc": nproc, "cpu_model": _cpu_model()}
+    what = ("oracle/_ref/libgsknn_opt.so (the reference's dgsknn_var1 with its AVX kernels, "
+            f"GSKNN_IC_NT={threads})" if have_ref else "oracle/gsknn_oracle.c -O3 -fopenmp")
+    nq, dt = measure(threads, have_ref)
+    nq1, dt1 = measure(1, have_ref)
+    out = {"value": p.m * nq / dt, "unit": "pairs/s", "cores": threads,
+           "kind": "reference" if have_ref else "port",
+           "sample": f"{nq} queries x {p.m} candidates (d={p.d}, k={p.k}) of the same call, "
+                     f"median of 3 runs of {dt:.2f} s, {what}",
+           "single_thread": {"value": p.m * nq1 / dt1, "unit": "pairs/s", "cores": 1,
+                             "sample": f"{nq1} queries x {p.m} candidates, median of 3 runs "
+                                       f"of {dt1:.2f} s"},
+           "nproc": nproc, "cpu_model": _cpu_model()}
+    if have_ref:
+        nqp, dtp = measure(threads, False)
+        out["port"] = {"value": p.m * nqp / dtp, "unit": "pairs/s", "cores": threads,
+                       "sample": f"{nqp} queries x {p.m} candidates, median of 3 runs of "
+                                 f"{dtp:.2f} s, oracle/gsknn_oracle.c -O3 -fopenmp"}
+    return out
 
 
 def copy_calibration(torch, dev, args):

@@ -135,6 +135,28 @@ def ref_opt_l1(XA, amap, XB, bmap, D, I):
         micro[0], rankk[0] = saved
 
 
+def ref_opt_gsknn(XA, XA2, amap, XB, XB2, bmap, D, I, threads=1):
+    """The reference's own optimized FP64 kNN (its dgsknn_var1 with the x86-64
+    AVX micro-kernels, frame/gsknn.c:883-1103): m = amap.size candidates, n =
+    bmap.size heap rows of stride k = D.shape[1] (<= 512).  `threads` is the
+    reference's GSKNN_IC_NT (read from the environment at each call)."""
+    r = ref_opt_lib()
+    if r is None:
+        raise FileNotFoundError(REF_OPT_PATH)
+    m, n, d, k = amap.size, bmap.size, XA.shape[1], D.shape[1]
+    assert XA.dtype == np.float64 and XB.dtype == np.float64 and k <= 512
+    prev = os.environ.get("GSKNN_IC_NT")
+    os.environ["GSKNN_IC_NT"] = str(int(threads))
+    try:
+        r.dgsknn_var1(ctypes.c_int(n), ctypes.c_int(m), ctypes.c_int(d), ctypes.c_int(k),
+                      _p(XB), _p(XB2), _p(bmap), _p(XA), _p(XA2), _p(amap), _p(D), _p(I))
+    finally:
+        if prev is None:
+            os.environ.pop("GSKNN_IC_NT", None)
+        else:
+            os.environ["GSKNN_IC_NT"] = prev
+
+
 def brute_l1(XA, amap, XB, bmap, k):
     """Long-double L1 brute force; rows ascending by (dist, idx)."""
     XA = np.ascontiguousarray(XA, dtype=np.float64)

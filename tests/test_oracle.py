@@ -171,3 +171,30 @@ def test_planted_mistakes_are_caught_by_fixture(fixtures, variant):
     np.testing.assert_array_equal(ok, Iref)  # the faithful restatement agrees
     bad = _py_variant(c, variant)
     assert not np.array_equal(bad, Iref)
+
+
+@pytest.mark.skipif(oracle.ref_opt_lib() is None, reason="reference library not built here")
+@pytest.mark.parametrize("d", [16, 64, 300])
+def test_oracle_matches_reference_optimized_path(d):
+    """The reference's production dgsknn_var1 (AVX micro-kernels, blocked in
+    256-dimension slices above d = 256; oracle/_ref/libgsknn_opt.so) returns
+    the oracle's neighbour sets on tie-free data, distances within its
+    different summation order's rounding, with 1 and with 4 threads."""
+    g = np.random.Generator(np.random.PCG64(70 + d))
+    X = g.standard_normal((3000, d))
+    X2 = oracle.sqnorm(X)
+    amap = g.integers(0, 3000, 2500).astype(np.int32)
+    bmap = g.integers(0, 3000, 300).astype(np.int32)
+    k = 24
+    Do = np.full((300, k), np.inf)
+    Io = np.full((300, k), -1, dtype=np.int32)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Do, Io)
+    Dos, Ios = oracle.sort_rows(Do, Io)
+    for threads in (1, 4):
+        Dr = np.full((300, k), 1.79e308)
+        Ir = np.full((300, k), -1, dtype=np.int32)
+        oracle.ref_opt_gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, threads=threads)
+        Drs, Irs = oracle.sort_rows(Dr, Ir)
+        np.testing.assert_array_equal(Irs, Ios)
+        scale = X2[bmap][:, None] + X2.max()
+        assert np.all(np.abs(Drs - Dos) <= 1e-12 * scale)
