@@ -492,3 +492,32 @@ def test_exact_scalar_candidate_form(dtype, d):
     Dr, Ir = ds.empty_heap(rows.size, k, dtype)
     oracle.gsknn(X, X2, amap, X, X2, bmap[rows].copy(), Dr, Ir, canonical=True)
     assert_rows(D.cpu().numpy()[rows], I.cpu().numpy()[rows], Dr, Ir)
+
+
+@pytest.mark.parametrize("k,nq", [(100, 300), (12, 5000), (300, 40)])
+def test_split_merge_paths_exact(k, nq):
+    """Exact mode with candidate splits and the two merge kernels: one wave
+    per query when the lists fit (k (1 + splits) <= 1024 entries), one thread
+    per query otherwise; rows pre-filled by a first call so the resident
+    maximum filters the lists."""
+    import torch
+    print(f"[gpu] split merge k={k} nq={nq}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(k + nq))
+    X = g.standard_normal((9000, 16))
+    X2 = oracle.sqnorm(X)
+    first = g.integers(0, 9000, 700).astype(np.int32)
+    amap = g.integers(0, 9000, 8001).astype(np.int32)
+    bmap = g.integers(0, 9000, nq).astype(np.int32)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((nq, k), float("inf"), dtype=torch.float64, device=dev)
+    I = torch.full((nq, k), -1, dtype=torch.int32, device=dev)
+    tb = torch.from_numpy(bmap).to(dev)
+    for am in (first, amap):
+        gsknn.dknn_dev(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2, tb, D, I,
+                       mode=gsknn.MODE_EXACT)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(nq, k)
+    oracle.gsknn(X, X2, first, X, X2, bmap, Dr, Ir, canonical=True)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir)

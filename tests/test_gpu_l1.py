@@ -169,3 +169,18 @@ def test_l1_full_size_sampled_rows():
     assert_bitwise(Dg[rows], Ig[rows], Dr, Ir)
     assert np.all(Ig >= 0)
     assert np.all(np.diff(Dg, axis=1) <= 0)  # rows come back sorted descending
+
+
+@pytest.mark.parametrize("dtype,k", [(np.float64, 300), (np.float32, 520)])
+def test_l1_large_k(dtype, k):
+    """Heaps too large for 64 queries per workgroup (32 or 16 instead)."""
+    print(f"[gpu] l1 large k={k} {np.dtype(dtype).name}", flush=True)
+    g = np.random.Generator(np.random.PCG64(k))
+    X = g.standard_normal((3000, 16)).astype(dtype)
+    amap = g.permutation(3000)[:2600].astype(np.int32)
+    bmap = g.permutation(3000)[:150].astype(np.int32)
+    D, I = empty(150, k, dtype)
+    Dr, Ir = D.copy(), I.copy()
+    Dg, Ig = gpu_l1_dev(X, amap, bmap, D, I)
+    oracle_l1(X, amap, bmap, Dr, Ir)
+    assert_bitwise(Dg, Ig, Dr, Ir)
