@@ -25,8 +25,9 @@ struct KnnArgs {
   const int *hrow;  // heap row of query j (nullptr: row j)
   int nsplit;       // candidate splits (grid.y); > 1 writes partial lists
   int cps;          // candidates per split
-  T *partD;         // [nsplit][k][n] ascending partial lists (nsplit > 1)
-  int *partI;
+  T *partD;         // ascending partial lists (nsplit > 1): [nsplit][k][n], or
+  int *partI;       // [n][nsplit][k] when part_qmajor
+  int part_qmajor;  // partial lists query-major (read by the wave merge)
 };
 
 enum Mode {
@@ -46,6 +47,9 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
 // Merge of split partial lists into the heap rows (gsknn_kernels.hip);
 // supported when a workgroup's heaps fit in LDS.
 bool knn_merge_supported(int k, int elem_bytes);
+// Whether a split launch's lists are merged one wave per query; its partial
+// lists are then laid out query-major (KnnArgs::part_qmajor).
+bool knn_merge_query_major(int n, int k, int nsplit);
 hipError_t launch_knn_merge(const KnnArgs<double> &a, hipStream_t s);
 hipError_t launch_knn_merge(const KnnArgs<float> &a, hipStream_t s);
 

@@ -122,7 +122,8 @@ __device__ void heap_load(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, in
 }
 
 // Heapsort the LDS heap ascending in place, then write it out: descending
-// into the resident row, or ascending into the partial list of a split.
+// into the resident row, or ascending into the partial list of a split
+// (query-major lists: each thread writes its k entries contiguously).
 template <typename T>
 __device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, int j, int split) {
   const int k = a.k;
@@ -141,7 +142,8 @@ __device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, i
     }
   } else {
     for (int t = 0; t < k; ++t) {
-      const size_t o = ((size_t)split * k + t) * a.n + j;
+      const size_t o = a.part_qmajor ? ((size_t)j * a.nsplit + split) * k + t
+                                     : ((size_t)split * k + t) * a.n + j;
       a.partD[o] = hd[t * QS + q];
       a.partI[o] = hi[t * QS + q];
     }

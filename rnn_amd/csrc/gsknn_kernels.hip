@@ -519,8 +519,9 @@ __global__ __launch_bounds__(QPB) void knn_merge_kernel(KnnArgs<T> a) {
   heap_store(b, hd, hi, QPB, q, j, 0);
 }
 
-// Merge for calls with few queries: one wave per query.  The 64 lanes load
-// the resident row and every partial list at once (up to 64 * R entries), drop
+// Merge for calls with few queries: one wave per query over query-major
+// partial lists.  The 64 lanes load the resident row and every partial list
+// at once (up to 64 * R entries, the lists contiguous in memory), drop
 // list entries that do not beat the resident row's maximum (the split
 // launches' sentinels among them), and extract the k smallest in canonical
 // order by k rounds of a wave-wide arg-min; identical (distance, index) pairs
@@ -545,8 +546,8 @@ __global__ __launch_bounds__(256) void knn_merge_wave_kernel(KnnArgs<T> a) {
       ed[r] = a.D[row + e];
       ei[r] = a.I[row + e];
     } else if (e < E) {
-      const int s = (e - k) / k, t = (e - k) % k;
-      const size_t o = ((size_t)s * k + t) * a.n + j;
+      // query-major lists: this query's entries are contiguous
+      const size_t o = (size_t)j * (E - k) + (e - k);
       ed[r] = a.partD[o];
       ei[r] = a.partI[o];
     }
@@ -980,10 +981,14 @@ hipError_t launch_exact(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
 // Calls with fewer queries than this merge with one wave per query.
 constexpr int kWaveMergeMaxN = 16384;
 
+bool merge_query_major(int n, int k, int nsplit) {
+  return n <= kWaveMergeMaxN && k * (1 + nsplit) <= 64 * 16;
+}
+
 template <typename T>
 hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
   const int E = a.k * (1 + a.nsplit);
-  if (a.n <= kWaveMergeMaxN && E <= 64 * 16) {
+  if (a.part_qmajor) {
     const dim3 grid((a.n + 3) / 4);
 #define GSKNN_WMERGE_CASE(R_)                                                 \
     if (E <= 64 * R_) {                                                       \
@@ -995,6 +1000,7 @@ hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
     GSKNN_WMERGE_CASE(8)
     GSKNN_WMERGE_CASE(16)
 #undef GSKNN_WMERGE_CASE
+    return hipErrorInvalidValue;  // query-major lists always fit (merge_query_major)
   }
   const int qpb = merge_qpb(a.k, (int)sizeof(T));
   const size_t lds = merge_lds(a.k, (int)sizeof(T), qpb);
@@ -1035,6 +1041,7 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
   if (p.qpb == 0) return hipErrorNotSupported;
   a.nsplit = p.nsplit;
   a.cps = p.cps;
+  a.part_qmajor = p.nsplit > 1 && merge_query_major(a.n, a.k, p.nsplit);
   if (p.nsplit > 1) {
     const size_t need = part_bytes(p, a.n, a.k, (int)sizeof(T));
     if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
@@ -1074,6 +1081,8 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
 }
 
 bool knn_merge_supported(int k, int elem_bytes) { return merge_qpb(k, elem_bytes) > 0; }
+
+bool knn_merge_query_major(int n, int k, int nsplit) { return merge_query_major(n, k, nsplit); }
 
 hipError_t launch_knn_merge(const KnnArgs<double> &a, hipStream_t s) { return launch_merge(a, s); }
 

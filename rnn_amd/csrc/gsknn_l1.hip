@@ -279,6 +279,7 @@ hipError_t launch_l1(KnnArgs<T> a, hipStream_t s, void *work, size_t work_bytes)
   if (p.qpb == 0) return hipErrorNotSupported;
   a.nsplit = p.nsplit;
   a.cps = p.cps;
+  a.part_qmajor = p.nsplit > 1 && knn_merge_query_major(a.n, a.k, p.nsplit);
   if (p.nsplit > 1) {
     const size_t need = l1_part_bytes(p, a.n, a.k, (int)sizeof(T));
     if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
