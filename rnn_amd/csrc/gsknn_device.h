@@ -10,6 +10,8 @@
 
 #include <limits.h>
 
+#include <type_traits>
+
 #include "gsknn_common.h"
 
 namespace gsknn {
@@ -151,145 +153,6 @@ __device__ void heap_store(const KnnArgs<T> &a, T *hd, int *hi, int QS, int q, i
 }
 
 // ---------------------------------------------------------------------------
-// Exact kernel: one thread per query, the reference oracle's arithmetic.
-// LDS: heap [k][QPB] | query rows [QPB][d+1] | candidate tile [d][CT] |
-//      candidate norms [CT] | candidate ids [CT]
-// ---------------------------------------------------------------------------
-template <typename T>
-struct ExactLayout {
-  size_t hd, hi, sq, sc, cn, ci, bytes;
-  // qreg: query rows live in registers (no LDS copy).
-  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT, bool qreg = false) {
-    hd = 0;
-    hi = al16(hd + sizeof(T) * (size_t)k * QPB);
-    sq = al16(hi + sizeof(int) * (size_t)k * QPB);
-    sc = al16(sq + (qreg ? 0 : sizeof(T) * (size_t)QPB * (d + 1)));
-    cn = al16(sc + sizeof(T) * (size_t)d * CT);
-    ci = al16(cn + sizeof(T) * (size_t)CT);
-    bytes = al16(ci + sizeof(int) * (size_t)CT);
-  }
-};
-
-// One workgroup's share of a launch: queries [qblock*QPB, +QPB) against the
-// candidates of split `split`.  Every thread of the workgroup must call it.
-// DQ > 0 (requires d <= DQ): each thread keeps its query row in registers and
-// scores 8 candidates per pass; DQ = 0 reads the row from LDS.  Either way a
-// pair's dot product is summed in p order with products and sums rounded
-// separately, the oracle's arithmetic.
-template <typename T, int QPB, int DQ = 0>
-__device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
-                            unsigned char *smem) {
-  const int k = a.k, d = a.d;
-  const ExactLayout<T> L(k, QPB, d, CT, DQ > 0);
-  T *hd = (T *)(smem + L.hd);
-  int *hi = (int *)(smem + L.hi);
-  T *sq = (T *)(smem + L.sq);
-  T *sc = (T *)(smem + L.sc);
-  T *cn = (T *)(smem + L.cn);
-  int *ci = (int *)(smem + L.ci);
-
-  const int q = threadIdx.x;
-  const int j = qblock * QPB + q;
-  const bool valid = j < a.n;
-  const int cbeg = split * a.cps;
-  const int cend = min(a.m, cbeg + a.cps);
-
-  T qn = T(0);
-  T rd = T(0);
-  int ri = 0;
-  T *myq = sq + (size_t)q * (d + 1);
-  T qreg[DQ > 0 ? DQ : 1];
-  if (valid) {
-    const T *xq = a.XB + (size_t)a.bmap[j] * d;
-    if (DQ > 0) {
-#pragma unroll
-      for (int p = 0; p < (DQ > 0 ? DQ : 1); ++p) qreg[p] = p < d ? xq[p] : T(0);
-    } else {
-      for (int p = 0; p < d; ++p) myq[p] = xq[p];
-    }
-    qn = a.XB2[a.bmap[j]];
-    heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
-    rd = hd[q];
-    ri = hi[q];
-  }
-
-  auto consider = [&](T s, int c) {
-    // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]]
-    const T dist = xadd(xadd(xmul(s, T(-2)), cn[c]), qn);
-    const int idx = ci[c];
-    if (cless(dist, idx, rd, ri)) {
-      sift_down(hd, hi, QPB, q, 0, k, dist, idx);
-      rd = hd[q];
-      ri = hi[q];
-    }
-  };
-
-  for (int c0 = cbeg; c0 < cend; c0 += CT) {
-    const int cnt = min(CT, cend - c0);
-    __syncthreads();
-    for (int e = q; e < cnt * d; e += QPB) {
-      const int c = e / d, p = e - c * d;
-      sc[p * CT + c] = a.XA[(size_t)a.amap[c0 + c] * d + p];
-    }
-    for (int e = q; e < cnt; e += QPB) {
-      const int id = a.amap[c0 + e];
-      cn[e] = a.XA2[id];
-      ci[e] = id;
-    }
-    __syncthreads();
-    if (!valid) continue;
-    int c = 0;
-    if constexpr (DQ > 0) {
-      for (; c + 8 <= cnt; c += 8) {
-        T s[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) s[u] = T(0);
-#pragma unroll
-        for (int p = 0; p < DQ; ++p) {
-          if (p < d) {
-            const T qp = qreg[p];
-            const T *cp = sc + p * CT + c;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) s[u] = xadd(s[u], xmul(cp[u], qp));
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) consider(s[u], c + u);
-      }
-      for (; c < cnt; ++c) {
-        T s0 = T(0);
-#pragma unroll
-        for (int p = 0; p < DQ; ++p)
-          if (p < d) s0 = xadd(s0, xmul(sc[p * CT + c], qreg[p]));
-        consider(s0, c);
-      }
-      continue;
-    }
-    for (; c + 4 <= cnt; c += 4) {
-      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-      for (int p = 0; p < d; ++p) {
-        const T qp = myq[p];
-        const T *cp = sc + p * CT + c;
-        s0 = xadd(s0, xmul(cp[0], qp));
-        s1 = xadd(s1, xmul(cp[1], qp));
-        s2 = xadd(s2, xmul(cp[2], qp));
-        s3 = xadd(s3, xmul(cp[3], qp));
-      }
-      consider(s0, c);
-      consider(s1, c + 1);
-      consider(s2, c + 2);
-      consider(s3, c + 3);
-    }
-    for (; c < cnt; ++c) {
-      T s = T(0);
-      for (int p = 0; p < d; ++p) s = xadd(s, xmul(sc[p * CT + c], myq[p]));
-      consider(s, c);
-    }
-  }
-  if (valid) heap_store(a, hd, hi, QPB, q, j, split);
-}
-
-// ---------------------------------------------------------------------------
 // Per-lane survivor queue for thread-per-query heaps.  A candidate that beats
 // the lane's root as of the last flush waits in the queue; when some lane's
 // queue would overflow, every lane applies its queue (and the group at hand)
@@ -372,6 +235,153 @@ __device__ __forceinline__ void queue_drain(LaneQueue<T, QR> &Q, bool valid, T *
 }
 
 constexpr int kLaneQueue = 8;
+
+// ---------------------------------------------------------------------------
+// Exact kernel: one thread per query, the reference oracle's arithmetic.
+// LDS: heap [k][QPB] | query rows [QPB][d+1] | candidate tile [d][CT] |
+//      candidate norms [CT] | candidate ids [CT]
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ExactLayout {
+  size_t hd, hi, sq, sc, cn, ci, bytes;
+  // qreg: query rows live in registers (no LDS copy).
+  __host__ __device__ ExactLayout(int k, int QPB, int d, int CT, bool qreg = false) {
+    hd = 0;
+    hi = al16(hd + sizeof(T) * (size_t)k * QPB);
+    sq = al16(hi + sizeof(int) * (size_t)k * QPB);
+    sc = al16(sq + (qreg ? 0 : sizeof(T) * (size_t)QPB * (d + 1)));
+    cn = al16(sc + sizeof(T) * (size_t)d * CT);
+    ci = al16(cn + sizeof(T) * (size_t)CT);
+    bytes = al16(ci + sizeof(int) * (size_t)CT);
+  }
+};
+
+// One workgroup's share of a launch: queries [qblock*QPB, +QPB) against the
+// candidates of split `split`.  Every thread of the workgroup must call it.
+// DQ > 0 (requires d <= DQ): each thread keeps its query row in registers and
+// scores 8 candidates per pass; DQ = 0 reads the row from LDS.  Either way a
+// pair's dot product is summed in p order with products and sums rounded
+// separately, the oracle's arithmetic.
+template <typename T, int QPB, int DQ = 0>
+__device__ void exact_block(const KnnArgs<T> &a, int CT, int qblock, int split,
+                            unsigned char *smem) {
+  const int k = a.k, d = a.d;
+  const ExactLayout<T> L(k, QPB, d, CT, DQ > 0);
+  T *hd = (T *)(smem + L.hd);
+  int *hi = (int *)(smem + L.hi);
+  T *sq = (T *)(smem + L.sq);
+  T *sc = (T *)(smem + L.sc);
+  T *cn = (T *)(smem + L.cn);
+  int *ci = (int *)(smem + L.ci);
+
+  const int q = threadIdx.x;
+  const int j = qblock * QPB + q;
+  const bool valid = j < a.n;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+
+  T qn = T(0);
+  T rd = T(0);
+  int ri = 0;
+  T *myq = sq + (size_t)q * (d + 1);
+  T qreg[DQ > 0 ? DQ : 1];
+#pragma unroll
+  for (int p = 0; p < (DQ > 0 ? DQ : 1); ++p) qreg[p] = T(0);
+  if (valid) {
+    const T *xq = a.XB + (size_t)a.bmap[j] * d;
+    if (DQ > 0) {
+#pragma unroll
+      for (int p = 0; p < (DQ > 0 ? DQ : 1); ++p) qreg[p] = p < d ? xq[p] : T(0);
+    } else {
+      for (int p = 0; p < d; ++p) myq[p] = xq[p];
+    }
+    qn = a.XB2[a.bmap[j]];
+    heap_load(a, hd, hi, QPB, q, j, a.nsplit > 1);
+    rd = hd[q];
+    ri = hi[q];
+  }
+
+  // Cs *= -2; Cs += XA2[alpha[i]]; Cs += XB2[beta[j]], then the lane queue.
+  // Every lane takes part (the queue's overflow test is wave-wide); lanes
+  // without a query compute on stale rows and offer nothing.
+  LaneQueue<T, kLaneQueue> Q;
+  queue_init(Q);
+  auto offer = [&](auto tc, const T *s, int c) __attribute__((always_inline)) {
+    constexpr int TC = decltype(tc)::value;
+    T dist[TC];
+    int id[TC];
+#pragma unroll
+    for (int u = 0; u < TC; ++u) {
+      dist[u] = xadd(xadd(xmul(s[u], T(-2)), cn[c + u]), qn);
+      id[u] = ci[c + u];
+    }
+    queue_offer<T, kLaneQueue, TC>(Q, dist, id, TC, valid, hd, hi, QPB, q, k, rd, ri);
+  };
+  typedef std::integral_constant<int, 1> I1;
+  typedef std::integral_constant<int, 4> I4;
+  typedef std::integral_constant<int, 8> I8;
+
+  for (int c0 = cbeg; c0 < cend; c0 += CT) {
+    const int cnt = min(CT, cend - c0);
+    __syncthreads();
+    for (int e = q; e < cnt * d; e += QPB) {
+      const int c = e / d, p = e - c * d;
+      sc[p * CT + c] = a.XA[(size_t)a.amap[c0 + c] * d + p];
+    }
+    for (int e = q; e < cnt; e += QPB) {
+      const int id = a.amap[c0 + e];
+      cn[e] = a.XA2[id];
+      ci[e] = id;
+    }
+    __syncthreads();
+    int c = 0;
+    if constexpr (DQ > 0) {
+      for (; c + 8 <= cnt; c += 8) {
+        T s[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] = T(0);
+#pragma unroll
+        for (int p = 0; p < DQ; ++p) {
+          if (p < d) {
+            const T qp = qreg[p];
+            const T *cp = sc + p * CT + c;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s[u] = xadd(s[u], xmul(cp[u], qp));
+          }
+        }
+        offer(I8{}, s, c);
+      }
+      for (; c < cnt; ++c) {
+        T s0[1] = {T(0)};
+#pragma unroll
+        for (int p = 0; p < DQ; ++p)
+          if (p < d) s0[0] = xadd(s0[0], xmul(sc[p * CT + c], qreg[p]));
+        offer(I1{}, s0, c);
+      }
+      continue;
+    }
+    for (; c + 4 <= cnt; c += 4) {
+      T s[4] = {T(0), T(0), T(0), T(0)};
+      for (int p = 0; p < d; ++p) {
+        const T qp = myq[p];
+        const T *cp = sc + p * CT + c;
+        s[0] = xadd(s[0], xmul(cp[0], qp));
+        s[1] = xadd(s[1], xmul(cp[1], qp));
+        s[2] = xadd(s[2], xmul(cp[2], qp));
+        s[3] = xadd(s[3], xmul(cp[3], qp));
+      }
+      offer(I4{}, s, c);
+    }
+    for (; c < cnt; ++c) {
+      T s[1] = {T(0)};
+      for (int p = 0; p < d; ++p) s[0] = xadd(s[0], xmul(sc[p * CT + c], myq[p]));
+      offer(I1{}, s, c);
+    }
+  }
+  queue_drain(Q, valid, hd, hi, QPB, q, k, rd, ri);
+  if (valid) heap_store(a, hd, hi, QPB, q, j, split);
+}
+
 
 // ---------------------------------------------------------------------------
 // Exact block routine, scalar-candidate form (d == D in {8, 16, 32, 64}).
