@@ -233,9 +233,15 @@ def precision_form(torch, dev, args):
     bf16_call()
     torch.cuda.synchronize()
     ms_b, _ = _timed(torch, stream, bf16_call, max(1, args.steps))
+    cand = gsknn.bf16_candidate_counts(work, p.m, p.n, p.d, p.k).double()
+    q = torch.quantile(cand, torch.tensor([0.5, 0.99], dtype=torch.float64, device=dev))
     out["bf16_rerank"] = {"ms": ms_b, "pairs_per_s": pairs / (ms_b * 1e-3),
                           "overflowed_queries": int(over.item()),
-                          "bf16_tflops_equiv": 2.0 * p.d * pairs / (ms_b * 1e-3) / 1e12}
+                          "bf16_tflops_equiv": 2.0 * p.d * pairs / (ms_b * 1e-3) / 1e12,
+                          "candidates_per_query": {"mean": float(cand.mean()),
+                                                   "p50": float(q[0]), "p99": float(q[1]),
+                                                   "max": float(cand.max())},
+                          "rerank_fp64_flop": 3.0 * p.d * float(cand.sum())}
     log(f"bf16+rerank {ms_b:.2f} ms, overflow {int(over.item())}")
 
     # Plain FP64 path (MFMA).

@@ -88,6 +88,13 @@ int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
                        void *work, size_t work_bytes, int *overflow,
                        void *stream);
 
+/* Candidate counts of the last gsknn_bf16_knn_dev call with this `work`
+ * (no counterpart in the reference; for reporting): at the returned byte
+ * offset, *nsplit x n ints, the number of candidates each query kept in each
+ * candidate split.  The re-rank reads up to the list capacity of them and
+ * re-scans a query whose count exceeded it. */
+size_t gsknn_bf16_count_offset(int m, int n, int d, int k, int *nsplit);
+
 /* Leaf sweep of one randomized-projection-tree iteration, as one persistent
  * launch.  perm holds the point ids grouped by leaf; items is an int array
  * [nitems][4] = {leaf start in perm, leaf size, first query of the block, 0}

@@ -247,6 +247,11 @@ extern "C" size_t gsknn_bf16_work_bytes(int m, int n, int d, int k) {
   return (m > 0 && n > 0 && d > 0 && k > 0) ? bf16_work_bytes(m, n, d, k) : 0;
 }
 
+extern "C" size_t gsknn_bf16_count_offset(int m, int n, int d, int k, int *nsplit) {
+  if (nsplit) *nsplit = 0;
+  return (m > 0 && n > 0 && d > 0 && k > 0) ? bf16_count_offset(m, n, d, k, nsplit) : 0;
+}
+
 extern "C" int gsknn_bf16_knn_dev(int m, int n, int d, int k, int NA, const float *XA,
                                   const void *prepA, const int *amap, int NB, const float *XB,
                                   const void *prepB, const int *bmap, double *D, int *I, int ldk,

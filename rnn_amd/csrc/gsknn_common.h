@@ -101,6 +101,8 @@ hipError_t launch_tree_level_keys(long long total, int nnodes, int d, int id_bit
 size_t bf16_prep_bytes(int N, int d);
 hipError_t launch_bf16_prep(int N, int d, const float *X, void *prep, hipStream_t s);
 size_t bf16_work_bytes(int m, int n, int d, int k);
+// Byte offset in the workspace of the [nsplit][n] candidate counts.
+size_t bf16_count_offset(int m, int n, int d, int k, int *nsplit);
 hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const float *XA,
                            const void *prepA, const int *amap, const float *XB, const void *prepB,
                            const int *bmap, double *D, int *I, int ldk, const int *hrow,

@@ -787,6 +787,16 @@ size_t bf16_work_bytes(int m, int n, int d, int k) {
   return bf16_work_layout(n, p.cap, p.nsplit, &a, &b, &c, &e);
 }
 
+size_t bf16_count_offset(int m, int n, int d, int k, int *nsplit) {
+  Bf16Plan p = plan_bf16(d, k);
+  if (p.ks == 0 || p.q == 0 || n <= 0 || m <= 0) return 0;
+  plan_splits(p, m, n);
+  size_t o_lst, o_cnt, o_u, o_over;
+  bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over);
+  if (nsplit) *nsplit = p.nsplit;
+  return o_cnt;
+}
+
 hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const float *XA,
                            const void *prepA, const int *amap, const float *XB, const void *prepB,
                            const int *bmap, double *D, int *I, int ldk, const int *hrow,

@@ -81,6 +81,8 @@ def _declare(L):
     L.gsknn_bf16_work_bytes.argtypes = [i32] * 4
     L.gsknn_bf16_work_bytes.restype = sz
     L.gsknn_bf16_knn_dev.argtypes = [i32] * 5 + [vp] * 3 + [i32] + [vp] * 5 + [i32, vp, vp, sz, vp, vp]
+    L.gsknn_bf16_count_offset.argtypes = [i32] * 4 + [ctypes.POINTER(i32)]
+    L.gsknn_bf16_count_offset.restype = sz
     L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_tree_level_keys_dev.argtypes = [ctypes.c_longlong, i32, i32, i32] + [vp] * 8
@@ -340,6 +342,15 @@ def bf16_prep_dev(X, stream=None):
 
 def bf16_work_bytes(m, n, d, k) -> int:
     return int(lib().gsknn_bf16_work_bytes(m, n, d, k))
+
+
+def bf16_candidate_counts(work, m, n, d, k):
+    """Per-query candidate counts (summed over candidate splits) of the last
+    bf16_knn_dev call that used the device byte tensor ``work``."""
+    ns = ctypes.c_int(0)
+    off = int(lib().gsknn_bf16_count_offset(m, n, d, k, ctypes.byref(ns)))
+    cnt = work[off:off + 4 * n * ns.value].view(dtype=__import__("torch").int32)
+    return cnt.view(ns.value, n).sum(dim=0)
 
 
 def bf16_knn_dev(XA, prepA, amap, XB, prepB, bmap, D, I, hrow=None, work=None, overflow=None,
