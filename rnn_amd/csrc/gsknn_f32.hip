@@ -225,15 +225,17 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
   // lane l writes element l of each, i.e. dim 4 s + l / 16 of candidate
   // cb * 16 + l % 16: every LDS store is 256 contiguous bytes (no bank
   // conflicts) and a load instruction reads 16 contiguous bytes of each of
-  // 16 candidate rows.  Wave w < CB, lanes 0..15 own the norm slots of
-  // candidate block w.
+  // 16 candidate rows.  Lanes 0..15 of wave w own the norm slots of
+  // candidate blocks w, w + W, ... (one block per wave when W >= CB).
   constexpr int BLK = KS * CB;
   static_assert(BLK % W == 0, "fragment blocks per wave");
   constexpr int BPW = BLK / W;
   const int si = lane & 15, skk = lane >> 4;
-  const bool nslot = w < CB && skk == 0;
+  constexpr int NPW = (CB + W - 1) / W;  // norm blocks per wave
   float stage[BPW];
-  float st_n = 0.f, st_x = 0.f;
+  float st_n[NPW], st_x[NPW];
+#pragma unroll
+  for (int r = 0; r < NPW; ++r) st_n[r] = st_x[r] = 0.f;
   // Ids of this thread's candidates: of the tile being staged (sid) and of
   // the tile after it (nid, requested a tile ahead so that no row load
   // waits on its amap entry).
@@ -262,13 +264,19 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
       const int p = 4 * (bidx / CB) + skk;
       stage[e] = a.XA[(size_t)stage_id(e) * d + (p < d ? p : d - 1)];
     }
-    if (nslot) {
-      int id = sid[0];
+    if (skk == 0) {
 #pragma unroll
-      for (int cb = 1; cb < CB; ++cb)
-        if (w == cb) id = sid[cb];
-      st_n = a.XA2[id];
-      st_x = a.xnA[cand_pos(t, w)];
+      for (int r = 0; r < NPW; ++r) {
+        const int nb = w + r * W;
+        if (nb < CB) {
+          int id = sid[0];
+#pragma unroll
+          for (int cb = 1; cb < CB; ++cb)
+            if (nb == cb) id = sid[cb];
+          st_n[r] = a.XA2[id];
+          st_x[r] = a.xnA[cand_pos(t, nb)];
+        }
+      }
     }
   };
   auto store_x = [&](int t) {
@@ -280,11 +288,17 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
       const bool ok = cbeg + t * C + cb * 16 + si < cend;
       tA[(size_t)bidx * 64 + lane] = (ok && p < d) ? stage[e] : 0.0f;
     }
-    if (nslot) {
-      const int slot = w * 16 + si;
-      const bool ok = cbeg + t * C + slot < cend;
-      tnc[slot] = ok ? st_n : __builtin_inff();
-      txn[slot] = ok ? st_x : 0.0f;
+    if (skk == 0) {
+#pragma unroll
+      for (int r = 0; r < NPW; ++r) {
+        const int nb = w + r * W;
+        if (nb < CB) {
+          const int slot = nb * 16 + si;
+          const bool ok = cbeg + t * C + slot < cend;
+          tnc[slot] = ok ? st_n[r] : __builtin_inff();
+          txn[slot] = ok ? st_x[r] : 0.0f;
+        }
+      }
     }
   };
 

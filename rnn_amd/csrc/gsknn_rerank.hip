@@ -287,14 +287,17 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   // candidate block cb); wave w stages blocks w * BPW .. w * BPW + BPW - 1 and
   // lane l writes 16-byte element l of each (dims 32 s + 8 (l / 16) .. + 8 of
   // candidate cb * 16 + l % 16): every LDS store is 1 KB contiguous (no bank
-  // conflicts).  Wave w < CB, lanes 0..15 own the norm slots of block w.
+  // conflicts).  Lanes 0..15 of wave w own the norm slots of candidate blocks
+  // w, w + W, ... (one block per wave when W >= CB).
   constexpr int BLK = KS * CB;
   static_assert(BLK % W == 0, "fragment blocks per wave");
   constexpr int BPW = BLK / W;
   const int si = lane & 15, skk = lane >> 4;
-  const bool nslot = w < CB && skk == 0;
+  constexpr int NPW = (CB + W - 1) / W;  // norm blocks per wave
   uint4 stage[BPW];
-  float st_n = 0.f, st_x = 0.f, st_e = 0.f;
+  float st_n[NPW], st_x[NPW], st_e[NPW];
+#pragma unroll
+  for (int r = 0; r < NPW; ++r) st_n[r] = st_x[r] = st_e[r] = 0.f;
   // Ids of this thread's candidates: of the tile being staged (sid) and of
   // the tile after it (nid, requested a tile ahead).
   typedef int idvec __attribute__((ext_vector_type(CB)));  // keeps selections in registers
@@ -322,14 +325,20 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
       const int ch = 4 * (bidx / CB) + skk;  // 16-byte chunk of the padded row
       stage[e] = *(const uint4 *)(xbA + (size_t)stage_id(e) * DP + 8 * ch);
     }
-    if (nslot) {
-      int id = sid[0];
+    if (skk == 0) {
 #pragma unroll
-      for (int cb = 1; cb < CB; ++cb)
-        if (w == cb) id = sid[cb];
-      st_n = (float)n64A[id];
-      st_x = xnA[id];
-      st_e = enA[id];
+      for (int r = 0; r < NPW; ++r) {
+        const int nb = w + r * W;
+        if (nb < CB) {
+          int id = sid[0];
+#pragma unroll
+          for (int cb = 1; cb < CB; ++cb)
+            if (nb == cb) id = sid[cb];
+          st_n[r] = (float)n64A[id];
+          st_x[r] = xnA[id];
+          st_e[r] = enA[id];
+        }
+      }
     }
   };
   auto store_x = [&](int t) {
@@ -339,12 +348,18 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
       const bool ok = cbeg + t * C + (bidx % CB) * 16 + si < cend;
       *(uint4 *)(tA + ((size_t)bidx * 64 + lane) * 8) = ok ? stage[e] : make_uint4(0, 0, 0, 0);
     }
-    if (nslot) {
-      const int slot = w * 16 + si;
-      const bool ok = cbeg + t * C + slot < cend;
-      tnc[slot] = ok ? st_n : __builtin_inff();
-      txn[slot] = ok ? st_x : 0.0f;
-      ten[slot] = ok ? st_e : 0.0f;
+    if (skk == 0) {
+#pragma unroll
+      for (int r = 0; r < NPW; ++r) {
+        const int nb = w + r * W;
+        if (nb < CB) {
+          const int slot = nb * 16 + si;
+          const bool ok = cbeg + t * C + slot < cend;
+          tnc[slot] = ok ? st_n[r] : __builtin_inff();
+          txn[slot] = ok ? st_x[r] : 0.0f;
+          ten[slot] = ok ? st_e[r] : 0.0f;
+        }
+      }
     }
   };
 
@@ -724,8 +739,17 @@ Bf16Plan plan_bf16(int d, int k) {
     }
   }
   if (p.ks == 0) return p;
-  const int opts[4][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
-  for (auto &o : opts) {
+  // Shapes (QB, W) in order of preference; GSKNN_BF16_CFG=<index> pins one
+  // (tuning).
+  constexpr int kOpts = 4;
+  const int opts[kOpts][2] = {{2, 4}, {1, 4}, {1, 2}, {1, 1}};
+  static const int pinned = [] {
+    const char *e = getenv("GSKNN_BF16_CFG");
+    return e ? atoi(e) : -1;
+  }();
+  for (int c = 0; c < kOpts; ++c) {
+    if (pinned >= 0 && pinned < kOpts && c != pinned) continue;
+    const int *o = opts[c];
     const int Q = o[1] * 16 * o[0];
     const size_t b = FilterLayout(p.ks, kCBb, Q, k).bytes;
     if (b <= kLds) {

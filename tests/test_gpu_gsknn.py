@@ -302,7 +302,10 @@ def _bf16_run(XA32, amap, XB32, bmap, D, I, hrow=None):
     return tD.cpu().numpy(), tI.cpu().numpy(), int(over.item())
 
 
-@pytest.mark.parametrize("d,k", [(256, 64), (128, 32), (32, 16), (50, 10)])
+# k = 600 and 700 take the 2-wave filter shape, k = 1100 the 1-wave shape
+# (heaps too large for 4 waves of queries in LDS).
+@pytest.mark.parametrize("d,k", [(256, 64), (128, 32), (32, 16), (50, 10), (256, 600),
+                                 (256, 1100), (64, 700)])
 def test_bf16_rerank_matches_fp64_oracle(d, k):
     # Low-rank generator of the precision workload (latent 10, noise 0.01).
     g = np.random.Generator(np.random.PCG64(d + k))
@@ -346,8 +349,10 @@ def test_bf16_carry_over_and_ties():
     assert_rows(Dg, Ig, Dr, Ir)
 
 
+# k = 700 takes the 2-wave filter shape and k = 1300 the 1-wave shape at
+# d = 24 (heaps too large for 4 waves of queries in LDS).
 @pytest.mark.parametrize("d,k,lattice", [(24, 17, False), (100, 32, False), (256, 64, False),
-                                         (12, 10, True)])
+                                         (12, 10, True), (24, 700, False), (24, 1300, False)])
 def test_sgsknn_fp32_mfma_rerank_bitwise(d, k, lattice):
     # n >= 4096 takes the FP32 MFMA candidate pass + re-rank in the oracle's
     # FP32 arithmetic; distances must still be the oracle's bits.
