@@ -159,20 +159,24 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     int v[CB];
   };
   double stage[BPW];
-  double stage_n = 0.0;
+  constexpr int NPW = (CB + W - 1) / W;  // norm/id blocks per wave
+  double stage_n[NPW];
+#pragma unroll
+  for (int r = 0; r < NPW; ++r) stage_n[r] = 0.0;
   auto cand_id = [&](int t) -> Ids {
     Ids r;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) r.v[cb] = a.amap[min(cbeg + t * C + cb * 16 + si, cend - 1)];
     return r;
   };
-  // Wave w < CB, lanes 0..15 own the norm and id slots of candidate block w.
-  const bool nslot = NB == 1 && w < CB && skk == 0;
-  auto own_id = [&](const Ids &ids) -> int {
+  // Lanes 0..15 of wave w own the norm and id slots of candidate blocks
+  // w, w + W, ... (one block per wave when W >= CB).
+  const bool nslot = NB == 1 && skk == 0;
+  auto own_id = [&](const Ids &ids, int nb) -> int {
     int r = ids.v[0];
 #pragma unroll
     for (int cb = 1; cb < CB; ++cb)
-      if (w == cb) r = ids.v[cb];
+      if (nb == cb) r = ids.v[cb];
     return r;
   };
   auto load_x = [&](const Ids &ids) {
@@ -183,7 +187,11 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       const int p = 4 * sb + skk;
       stage[e] = a.XA[(size_t)ids.v[cb] * d + (p < d ? p : d - 1)];
     }
-    if (nslot) stage_n = a.XA2[own_id(ids)];
+    if (nslot) {
+#pragma unroll
+      for (int r = 0; r < NPW; ++r)
+        if (w + r * W < CB) stage_n[r] = a.XA2[own_id(ids, w + r * W)];
+    }
   };
   auto store_x = [&](int buf, int t, const Ids &ids) {
     double *dst = tA + buf * TILE;
@@ -198,10 +206,16 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     if (nslot) {
       // An empty slot gets distance +inf and index INT_MAX: it loses to
       // every row entry, including (+inf, -1) placeholders.
-      const int slot = w * 16 + si;
-      const bool ok = cbeg + t * C + slot < cend;
-      tn[slot] = ok ? stage_n : __builtin_inf();
-      ti[slot] = ok ? own_id(ids) : INT_MAX;
+#pragma unroll
+      for (int r = 0; r < NPW; ++r) {
+        const int nb = w + r * W;
+        if (nb < CB) {
+          const int slot = nb * 16 + si;
+          const bool ok = cbeg + t * C + slot < cend;
+          tn[slot] = ok ? stage_n[r] : __builtin_inf();
+          ti[slot] = ok ? own_id(ids, nb) : INT_MAX;
+        }
+      }
     }
   };
   // NB = 2: this lane's tile candidate (lane % C), id two tiles ahead and

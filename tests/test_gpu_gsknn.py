@@ -526,3 +526,28 @@ def test_split_merge_paths_exact(k, nq):
     oracle.gsknn(X, X2, first, X, X2, bmap, Dr, Ir, canonical=True)
     oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
     assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir)
+
+
+@pytest.mark.parametrize("d,k", [(64, 500), (40, 420), (100, 700)])
+def test_mfma_large_k_single_wave_shape(d, k):
+    """FP64 MFMA with heaps so large that only one wave of 16 queries fits a
+    workgroup's LDS (fewer waves than candidate blocks per tile)."""
+    import torch
+    print(f"[gpu] mfma large k d={d} k={k}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(d * 1000 + k))
+    X = g.standard_normal((5000, d))
+    X2 = oracle.sqnorm(X)
+    amap = g.integers(0, 5000, 3001).astype(np.int32)
+    bmap = g.integers(0, 5000, 200).astype(np.int32)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((200, k), float("inf"), dtype=torch.float64, device=dev)
+    I = torch.full((200, k), -1, dtype=torch.int32, device=dev)
+    gsknn.dknn_dev(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2,
+                   torch.from_numpy(bmap).to(dev), D, I, mode=gsknn.MODE_MFMA)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(200, k)
+    oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is],
+                tol=TOL64)
