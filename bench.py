@@ -397,17 +397,18 @@ def c1_small(torch, dev, reps=200):
         I.fill_(-1)
         pair()
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    total = 0.0
-    for _ in range(reps):
+    # Every repetition is enqueued before the first sync, so each event pair
+    # brackets device work only (no host launch latency inside the interval).
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for e0, e1 in ev:
         D.fill_(float("inf"))
         I.fill_(-1)
         e0.record(stream)
         pair()
         e1.record(stream)
-        torch.cuda.synchronize()
-        total += e0.elapsed_time(e1)
+    torch.cuda.synchronize()
+    total = sum(e0.elapsed_time(e1) for e0, e1 in ev)
     us = total * 1e3 / (2 * reps)
     pairs = float(p.m) * p.n
     B = (p.m + p.n) * (8 * p.d + 8 + 4) + 2 * p.n * p.k * (8 + 4)
@@ -485,7 +486,8 @@ def main():
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
     ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma", "tile"],
+                    help="FP64 contraction (tile: MFMA with the LDS-tile kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -520,7 +522,8 @@ def main():
     t0 = time.perf_counter()
     p = ds.config2()
     log(f"data N={p.XA.shape[0]} d={p.d} m={p.m} n={p.n} k={p.k} ({time.perf_counter() - t0:.1f}s)")
-    mode = {"auto": gsknn.MODE_AUTO, "exact": gsknn.MODE_EXACT, "mfma": gsknn.MODE_MFMA}[args.mode]
+    mode = {"auto": gsknn.MODE_AUTO, "exact": gsknn.MODE_EXACT, "mfma": gsknn.MODE_MFMA,
+            "tile": gsknn.MODE_MFMA_TILE}[args.mode]
 
     X = torch.from_numpy(p.XA).to(dev)
     X2 = torch.from_numpy(p.XA2).to(dev)

@@ -27,13 +27,17 @@ extern "C" {
 
 /* Contraction selection.
  * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
- * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA.
+ * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA
+ * (for d <= 64 the form that streams candidates into registers per wave,
+ * when the heaps fit its LDS); MFMA_TILE = FP64 MFMA with the LDS-tile
+ * kernel only (same results within the FP64 tolerance).
  * FP32: AUTO/MFMA = FP32 MFMA candidate pass + re-rank in the oracle's FP32
  * arithmetic when d >= 8 and m >= 64, else the exact VALU kernel; EXACT =
  * always the VALU kernel.  Every FP32 mode returns the oracle's distance bits. */
 #define GSKNN_MODE_AUTO 0
 #define GSKNN_MODE_EXACT 1
 #define GSKNN_MODE_MFMA 2
+#define GSKNN_MODE_MFMA_TILE 3
 
 /* Error codes. */
 #define GSKNN_OK 0

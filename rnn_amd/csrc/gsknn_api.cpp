@@ -113,7 +113,7 @@ extern "C" int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA, cons
                               const int *amap, const double *XB, const double *XB2,
                               const int *bmap, double *D, int *I, int ldk, const int *hrow,
                               int mode, void *work, size_t work_bytes, void *stream) {
-  if (mode < 0 || mode > 2) return set_err(GSKNN_EINVAL, "gsknn_dknn_dev: mode %d", mode);
+  if (mode < 0 || mode > 3) return set_err(GSKNN_EINVAL, "gsknn_dknn_dev: mode %d", mode);
   return knn_dev("gsknn_dknn_dev", m, n, d, k, XA, XA2, amap, XB, XB2, bmap, D, I, ldk, hrow,
                  mode, work, work_bytes, stream);
 }
@@ -122,7 +122,7 @@ extern "C" int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA, const
                               const int *amap, const float *XB, const float *XB2,
                               const int *bmap, float *D, int *I, int ldk, const int *hrow,
                               int mode, void *work, size_t work_bytes, void *stream) {
-  if (mode < 0 || mode > 2) return set_err(GSKNN_EINVAL, "gsknn_sknn_dev: mode %d", mode);
+  if (mode < 0 || mode > 3) return set_err(GSKNN_EINVAL, "gsknn_sknn_dev: mode %d", mode);
   return knn_dev("gsknn_sknn_dev", m, n, d, k, XA, XA2, amap, XB, XB2, bmap, D, I, ldk, hrow,
                  mode, work, work_bytes, stream);
 }

@@ -34,6 +34,7 @@ enum Mode {
   MODE_AUTO = 0,   // FP64: MFMA when d >= 32, else exact VALU
   MODE_EXACT = 1,  // VALU, the reference oracle's arithmetic (bit-exact)
   MODE_MFMA = 2,   // FP64 MFMA contraction
+  MODE_MFMA_TILE = 3,  // FP64 MFMA, the LDS-tile kernel only (FP32: as MODE_MFMA)
 };
 
 // Launchers (gsknn_kernels.hip).  Return hipSuccess or the launch error.

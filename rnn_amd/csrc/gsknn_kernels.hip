@@ -482,6 +482,286 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 }
 
 // ---------------------------------------------------------------------------
+// Direct form of the FP64 MFMA kernel: every wave streams the candidates on
+// its own, global -> registers, already in the MFMA operand layout.  There is
+// no LDS tile and no workgroup barrier inside the candidate loop, so one
+// wave's filter and heap work overlaps the MFMAs of the other wave on its
+// SIMD instead of holding up a workgroup at a barrier.
+//
+// Lane l supplies k-index g = l / 16 of every 16x16x4 product.  The dims are
+// dealt to the four lane groups in contiguous runs: group g owns dims
+// [g KS, min((g + 1) KS, d)) and its k-step s is dim st + s with
+// st = min(g KS, d - KS), for the candidate operand and the query operand
+// alike (the dot product is the same sum, taken in another order).  A run
+// moved back from g KS overlaps dims of the groups before it; those steps
+// get a zero query coordinate, so every dim enters the sum exactly once and
+// every lane reads KS contiguous in-bounds doubles of each candidate row.
+// Candidate operands of tile t + 1 are loaded into the registers of tile t
+// as its MFMAs consume them, amap entries two tiles ahead, norms one tile
+// ahead.
+// The result layout, filter, survivor queue and heap updates are those of
+// knn_mfma_f64_kernel; the candidate norms and ids of a lane's result values
+// come from the lanes that loaded them, by lane shuffles.
+// ---------------------------------------------------------------------------
+template <int KS, int QB, int CB, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void knn_mfma_f64_direct_kernel(
+    KnnArgs<double> a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
+  static_assert(QB <= 2, "query block fits one bit");
+  typedef int idv __attribute__((ext_vector_type(CB * 4)));
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k, d = a.d;
+  double *hd = (double *)smem;
+  int *hi = (int *)(smem + al16(sizeof(double) * (size_t)k * Q));
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6, grp = lane >> 4, cl = lane & 15;
+  const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const bool sentinel = a.nsplit > 1;
+
+  if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
+
+  const int own0 = grp * KS, own1 = min(own0 + KS, d);  // dims this lane's group owns
+  const int st = min(own0, d - KS);                        // first dim it reads
+  double bq[QB][KS];
+  double qn[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + cl;
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const int jj = qv[qb] ? j : a.n - 1;
+    const double *xq = a.XB + (size_t)a.bmap[jj] * d;
+    qn[qb] = qv[qb] ? a.XB2[a.bmap[jj]] : 0.0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int p = st + s;
+      bq[qb][s] = (qv[qb] && p >= own0 && p < own1) ? xq[p] : 0.0;
+    }
+  }
+
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
+  struct Ids {
+    int v[CB];
+  };
+  // amap entries of this lane's candidates cb * 16 + l % 16 of tile t.
+  auto cand_id = [&](int t) -> Ids {
+    Ids r;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) r.v[cb] = a.amap[min(cbeg + t * C + cb * 16 + cl, cend - 1)];
+    return r;
+  };
+  auto slot_ok = [&](int t, int cb) -> bool { return cbeg + t * C + cb * 16 + cl < cend; };
+  double xb[KS][CB];  // candidate operands of the current tile
+  auto row = [&](const Ids &ids, int cb) -> const double * {
+    return a.XA + (size_t)ids.v[cb] * d + st;
+  };
+  auto load_op = [&](const double *r, int s) -> double { return r[s]; };
+
+  double cn[CB];  // norms of this lane's candidates in the current tile
+  int ci[CB];     // their ids (INT_MAX for empty slots)
+  Ids id_a{}, id_b{};
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) {
+    cn[cb] = __builtin_inf();
+    ci[cb] = INT_MAX;
+  }
+  if (ntiles > 0) {
+    const Ids id0 = cand_id(0);
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const double *r = row(id0, cb);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) xb[s][cb] = load_op(r, s);
+      const bool ok = slot_ok(0, cb);
+      cn[cb] = ok ? a.XA2[id0.v[cb]] : __builtin_inf();
+      ci[cb] = ok ? id0.v[cb] : INT_MAX;
+    }
+    id_a = ntiles > 1 ? cand_id(1) : id0;
+    id_b = ntiles > 2 ? cand_id(2) : id_a;
+  }
+  __syncthreads();  // heaps loaded
+
+  // Filter threshold of a query row, as in knn_mfma_f64_kernel: a pair with
+  // fl(t + qn) <= r has t <= fl(fl(r - qn) + 2^-50 (|r| + |qn|) + 2^-1020).
+  auto root_thr = [&](int qb) -> double {
+    if (!qv[qb]) return -__builtin_inf();
+    const double r = hd[ql[qb]];
+    const double m = xadd(xmul(0x1p-50, xadd(__builtin_fabs(r), __builtin_fabs(qn[qb]))), 0x1p-1020);
+    return xadd(xadd(r, -qn[qb]), m);
+  };
+  double thr[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
+
+  constexpr int QR = 6;
+  double pq_d[QR];
+  unsigned pq_i[QR];
+  int pq_n = 0;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+    pq_d[r] = 0.0;
+    pq_i[r] = 0u;
+  }
+  dbl4 acc[QB][CB];
+  // Result value b = cb * 4 + v of a lane is the pair (query row ql[qb],
+  // candidate cb * 16 + grp + 4 v); its exact distance fl(t + qn).
+  auto survivor = [&](int qb, int b, const idv cid, double &dist, int &idx) __attribute__((always_inline)) {
+    double t = acc[qb][0][0];
+    idx = cid[0];
+#pragma unroll
+    for (int bb = 1; bb < CB * 4; ++bb)
+      if (b == bb) {
+        t = acc[qb][bb >> 2][bb & 3];
+        idx = cid[bb];
+      }
+    dist = xadd(t, qn[qb]);
+  };
+  auto flush = [&](const unsigned *pass, const idv cid) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+      if (grp == g) {
+#pragma unroll
+        for (int r = 0; r < QR; ++r) {
+          if (r < pq_n) {
+            const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
+            const int id = (int)(pq_i[r] & 0x7fffffffu);
+            if (cless(pq_d[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, pq_d[r], id);
+          }
+        }
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass ? pass[qb] : 0u;
+          const int slot = ql[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            double dist;
+            int idx;
+            survivor(qb, b, cid, dist, idx);
+            if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+    pq_n = 0;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
+  };
+
+  for (int t = 0; t < ntiles; ++t) {
+    const Ids id_c = (t + 3 < ntiles) ? cand_id(t + 3) : id_b;
+    double nn[CB];  // norms of tile t + 1 (its ids are id_a)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) nn[cb] = a.XA2[id_a.v[cb]];
+    const double *nr[CB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) nr[cb] = row(id_a, cb);
+
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // Each operand register is refilled with tile t + 1's value once the
+    // MFMAs of tile t have read it (on the last tile the refill reads valid
+    // rows and is never used).
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const double av = xb[s][cb];
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+        xb[s][cb] = load_op(nr[cb], s);
+      }
+    }
+
+    // Epilogue: one FMA and one compare per pair; wave masks in scalar
+    // registers; per-lane bits, exact distances and ids only when a pair
+    // passes.
+    double ctn[CB * 4];
+#pragma unroll
+    for (int b = 0; b < CB * 4; ++b) ctn[b] = __shfl(cn[b >> 2], grp + 4 * (b & 3));
+    unsigned long long msk[QB][CB * 4];
+    unsigned long long anym = 0;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double tt = __builtin_fma(acc[qb][cb][v], -2.0, ctn[cb * 4 + v]);
+          acc[qb][cb][v] = tt;
+          msk[qb][cb * 4 + v] = __ballot(tt <= thr[qb]);
+          anym |= msk[qb][cb * 4 + v];
+        }
+    if (anym) {
+      unsigned pass[QB];
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        unsigned long long aq = 0ull;
+#pragma unroll
+        for (int b = 0; b < CB * 4; ++b) aq |= msk[qb][b];
+        pass[qb] = 0u;
+        if (aq != 0ull) {
+#pragma unroll
+          for (int b = 0; b < CB * 4; ++b) pass[qb] |= (unsigned)((msk[qb][b] >> lane) & 1ull) << b;
+        }
+      }
+      idv cid;
+#pragma unroll
+      for (int b = 0; b < CB * 4; ++b) cid[b] = __shfl(ci[b >> 2], grp + 4 * (b & 3));
+      int npass = 0;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
+      if (__any(pq_n + npass > QR)) {
+        flush(pass, cid);
+      } else {
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            double dist;
+            int idx;
+            survivor(qb, b, cid, dist, idx);
+#pragma unroll
+            for (int r = 0; r < QR; ++r)
+              if (pq_n == r) {
+                pq_d[r] = dist;
+                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+              }
+            ++pq_n;
+          }
+        }
+      }
+    }
+
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const bool ok = t + 1 < ntiles && slot_ok(t + 1, cb);
+      cn[cb] = ok ? nn[cb] : __builtin_inf();
+      ci[cb] = ok ? id_a.v[cb] : INT_MAX;
+    }
+    id_a = id_b;
+    id_b = id_c;
+  }
+
+  if (__any(pq_n > 0)) flush(nullptr, idv{});
+  __syncthreads();
+  if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
+}
+
+// ---------------------------------------------------------------------------
 // Merge of split partial lists into the resident rows: one thread per query.
 // Each partial list is ascending, so the scan of a list stops at its first
 // entry that does not beat the root.  Sentinel entries (copies of the
@@ -743,6 +1023,8 @@ struct Plan {
   int qpb = 64;  // queries per workgroup
   int ct = 0;    // exact: candidates per LDS tile (0: scalar-candidate form)
   int ks = 0, qb = 0, w = 0, nb = 1;  // mfma
+  bool direct = false;                // mfma: the direct form (no LDS tile)
+  int per_cu = 0;                     // direct form: resident workgroups per CU
   size_t lds = 0;
   int gx = 0, nsplit = 1, cps = 0;
 };
@@ -763,6 +1045,9 @@ int mfma_ks(int d) {
 // Candidate blocks of 16 per tile: two for d <= 64, one above, where the
 // query fragments already take most of the registers.
 constexpr int mfma_cb(int ks) { return ks >= 32 ? 1 : 2; }
+
+// LDS of the direct form: the Q heap rows only.
+size_t mfma_direct_lds(int Q, int k) { return al16(sizeof(double) * (size_t)k * Q) + sizeof(int) * (size_t)k * Q; }
 
 size_t merge_lds(int k, int elem, int qpb) {
   return al16((size_t)elem * k * qpb) + sizeof(int) * (size_t)k * qpb;
@@ -837,11 +1122,35 @@ int mfma_blocks_per_cu(int ks, int cfg) {
   return b;
 }
 
+// Direct-form workgroups (one wave each) per CU that registers allow; 0 for
+// an instantiation that spills.
+template <int KS>
+int mfma_direct_reg_blocks() {
+  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, 2, 2, 1>;
+  hipFuncAttributes at;
+  if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, 64, 0) != hipSuccess || b <= 0) b = 8;
+  return b;
+}
+
+int mfma_direct_blocks_per_cu(int ks) {
+  static std::atomic<int> cache[3];
+  const int ki = ks == 4 ? 0 : ks == 8 ? 1 : 2;
+  int b = cache[ki].load(std::memory_order_relaxed);  // blocks + 1
+  if (b > 0) return b - 1;
+  b = ks == 4 ? mfma_direct_reg_blocks<4>() : ks == 8 ? mfma_direct_reg_blocks<8>() : mfma_direct_reg_blocks<16>();
+  cache[ki].store(b + 1, std::memory_order_relaxed);
+  if (verbose()) fprintf(stderr, "gsknn: direct ks=%d: %d workgroups/CU by registers\n", ks, b);
+  return b;
+}
+
 Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   Plan p;
   const bool f64 = elem == 8;
   const int ks = f64 ? mfma_ks(d) : 0;
-  const bool mfma = f64 && ks > 0 && (mode == MODE_MFMA || (mode == MODE_AUTO && d >= 32));
+  const bool mfma = f64 && ks > 0 &&
+                    (mode == MODE_MFMA || mode == MODE_MFMA_TILE || (mode == MODE_AUTO && d >= 32));
   int tile_c = 0;
   if (mfma) {
     p.kind = 1;
@@ -872,11 +1181,35 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
         p.lds = b;
       }
     }
-    if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
     tile_c = 16 * mfma_cb(ks);
+    // The direct form (d <= 64): one wave of 32 queries per workgroup, only
+    // the heap rows in LDS.  Taken when its score (resident waves x QB = 2)
+    // is at least the LDS-tile kernel's.  GSKNN_MFMA_DIRECT=0 turns it off
+    // (tuning).
+    static const bool direct_on = [] {
+      const char *e = getenv("GSKNN_MFMA_DIRECT");
+      return !(e && *e == '0');
+    }();
+    if (direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 16 && d >= ks) {
+      const size_t b = mfma_direct_lds(32, k);
+      if (b <= kLdsMax) {
+        const int blocks = std::min((int)(kLdsMax / b), mfma_direct_blocks_per_cu(ks));
+        if (blocks > 0 && 2 * blocks >= best) {
+          best = 2 * blocks;
+          p.direct = true;
+          p.qb = 2;
+          p.w = 1;
+          p.nb = 0;
+          p.qpb = 32;
+          p.lds = b;
+          p.per_cu = blocks;
+        }
+      }
+    }
+    if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
     if (verbose())
-      fprintf(stderr, "gsknn: mfma plan d=%d k=%d: ks=%d qb=%d w=%d nb=%d lds=%zu score=%d\n", d,
-              k, ks, p.qb, p.w, p.nb, p.lds, best);
+      fprintf(stderr, "gsknn: mfma plan d=%d k=%d: ks=%d qb=%d w=%d nb=%d direct=%d lds=%zu score=%d\n",
+              d, k, ks, p.qb, p.w, p.nb, (int)p.direct, p.lds, best);
   }
   if (p.kind == 0) {
     // d in {8, 16, 32, 64}: the scalar-candidate form (heaps only in LDS).
@@ -907,6 +1240,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   // waits on scalar loads and wants every resident wave LDS allows, over
   // two rounds of the chip.
   int target = kTargetBlocks;
+  if (p.direct) target = 256 * p.per_cu;
   if (p.kind == 0 && p.ct == 0) {
     const int per_cu = (int)std::min<size_t>(16, kLdsMax / p.lds);
     target = 2 * 256 * per_cu;
@@ -930,6 +1264,25 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     p.cps = m;
   }
   return p;
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, size_t bytes);
+
+template <int KS>
+hipError_t launch_mfma_direct_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
+  const dim3 grid(p.gx, p.nsplit);
+  hipError_t e = hipSuccess;
+#define GSKNN_DIRECT_CASE(W_)                                                 \
+  if (p.w == W_) {                                                            \
+    auto kern = knn_mfma_f64_direct_kernel<KS, 2, 2, W_>;                     \
+    if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
+    hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
+    return hipGetLastError();                                                 \
+  }
+  GSKNN_DIRECT_CASE(1)
+#undef GSKNN_DIRECT_CASE
+  return hipErrorNotSupported;
 }
 
 template <typename K>
@@ -1064,7 +1417,18 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
     a.partI = (int *)((char *)work + al16(ents * sizeof(T)));
   }
   hipError_t e;
-  if (p.kind == 1) {
+  if (p.kind == 1 && p.direct) {
+    if constexpr (sizeof(T) == 8) {
+      switch (p.ks) {
+        case 4: e = launch_mfma_direct_ks<4>(p, a, s); break;
+        case 8: e = launch_mfma_direct_ks<8>(p, a, s); break;
+        case 16: e = launch_mfma_direct_ks<16>(p, a, s); break;
+        default: return hipErrorNotSupported;
+      }
+    } else {
+      return hipErrorNotSupported;
+    }
+  } else if (p.kind == 1) {
     if constexpr (sizeof(T) == 8) {
       switch (p.ks) {
         case 4: e = launch_mfma_ks<4>(p, a, s); break;

@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libgsknn.so")
 
-MODE_AUTO, MODE_EXACT, MODE_MFMA = 0, 1, 2
+MODE_AUTO, MODE_EXACT, MODE_MFMA, MODE_MFMA_TILE = 0, 1, 2, 3
 
 _lib = None
 

@@ -551,3 +551,58 @@ def test_mfma_large_k_single_wave_shape(d, k):
     Is = _sorted(Dr, Ir)[1]
     assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is],
                 tol=TOL64)
+
+
+def _mfma_case(X, amaps, bmap, k, mode):
+    import torch
+    dev = torch.device("cuda")
+    X2 = oracle.sqnorm(X)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    nq = bmap.size
+    D = torch.full((nq, k), float("inf"), dtype=torch.float64, device=dev)
+    I = torch.full((nq, k), -1, dtype=torch.int32, device=dev)
+    tb = torch.from_numpy(bmap).to(dev)
+    for am in amaps:
+        gsknn.dknn_dev(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2, tb, D, I, mode=mode)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(nq, k)
+    for am in amaps:
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    return D.cpu().numpy(), I.cpu().numpy(), Dr, Ir, X2
+
+
+@pytest.mark.parametrize("form", ["direct", "tile"])
+@pytest.mark.parametrize("d,k,nq", [(64, 32, 1000), (64, 1, 700), (63, 200, 500), (50, 400, 300),
+                                    (33, 17, 37), (17, 32, 900), (5, 9, 400), (3, 8, 300),
+                                    (64, 64, 20)])
+def test_mfma_direct_and_tile_forms(form, d, k, nq):
+    """FP64 MFMA in both forms: the direct form (each wave streams candidate
+    operands into registers; dims dealt to lane groups in runs moved back to
+    stay inside the row) and the LDS-tile kernel.  Partial last tile (3001
+    candidates), pre-filled rows, candidate splits for few queries, dims that
+    leave some lane runs overlapping (d = 5, 17, 33, 50, 63) and d = 3, below
+    the direct form's run length (falls back to the tile kernel)."""
+    print(f"[gpu] mfma form={form} d={d} k={k} nq={nq}", flush=True)
+    g = np.random.Generator(np.random.PCG64(d * 7919 + k * 31 + nq))
+    X = g.standard_normal((6000, d))
+    first = g.integers(0, 6000, 500).astype(np.int32)
+    amap = g.integers(0, 6000, 3001).astype(np.int32)
+    bmap = g.integers(0, 6000, nq).astype(np.int32)
+    mode = gsknn.MODE_MFMA if form == "direct" else gsknn.MODE_MFMA_TILE
+    Dg, Ig, Dr, Ir, X2 = _mfma_case(X, (first, amap), bmap, k, mode)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[np.maximum(Is, 0)], tol=TOL64)
+
+
+@pytest.mark.parametrize("form", ["direct", "tile"])
+def test_mfma_forms_lattice_ties(form):
+    """Integer lattice at d = 40: every MFMA dot product is exact, so rows
+    hold exact ties and the lower index must win, bit-identical distances."""
+    print(f"[gpu] mfma lattice form={form}", flush=True)
+    g = np.random.Generator(np.random.PCG64(4040))
+    X = g.integers(-3, 4, (5000, 40)).astype(np.float64)
+    amap = g.integers(0, 5000, 4000).astype(np.int32)
+    bmap = g.integers(0, 5000, 600).astype(np.int32)
+    mode = gsknn.MODE_MFMA if form == "direct" else gsknn.MODE_MFMA_TILE
+    Dg, Ig, Dr, Ir, _ = _mfma_case(X, (amap,), bmap, 24, mode)
+    assert_rows(Dg, Ig, Dr, Ir)
