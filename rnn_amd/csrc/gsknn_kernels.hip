@@ -488,24 +488,33 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 // wave's filter and heap work overlaps the MFMAs of the other wave on its
 // SIMD instead of holding up a workgroup at a barrier.
 //
-// Lane l supplies k-index g = l / 16 of every 16x16x4 product.  The dims are
-// dealt to the four lane groups in contiguous runs: group g owns dims
-// [g KS, min((g + 1) KS, d)) and its k-step s is dim st + s with
-// st = min(g KS, d - KS), for the candidate operand and the query operand
-// alike (the dot product is the same sum, taken in another order).  A run
-// moved back from g KS overlaps dims of the groups before it; those steps
-// get a zero query coordinate, so every dim enters the sum exactly once and
-// every lane reads KS contiguous in-bounds doubles of each candidate row.
-// Candidate operands of tile t + 1 are loaded into the registers of tile t
-// as its MFMAs consume them, amap entries two tiles ahead, norms one tile
-// ahead.
+// Lane l supplies k-index g = l / 16 of every 16x16x4 product; which dim
+// k-step s of group g carries is free, as long as the candidate operand and
+// the query operand agree (the dot product is the same sum, taken in another
+// order).  Two deals:
+//  * LINE (d = 4 KS): k-steps 2j and 2j + 1 of group g are dims 8j + 2g and
+//    8j + 2g + 1, so one 16-byte load per lane covers 64 contiguous bytes of
+//    each of the tile's 16 candidate rows, and two consecutive step pairs
+//    cover whole 128-byte lines.  The refills of tile t + 1 are issued in
+//    groups of four k-steps right behind the MFMAs that free their
+//    registers (a scheduling barrier keeps them there): every group reads
+//    whole lines while they are in L1, and each refill has a full tile of
+//    MFMAs before it is needed.
+//  * otherwise, contiguous runs: group g owns dims [g KS, min((g + 1) KS, d))
+//    and its k-step s is dim st + s with st = min(g KS, d - KS).  A run moved
+//    back from g KS overlaps dims of the groups before it; those steps get a
+//    zero query coordinate, so every dim enters the sum exactly once and
+//    every lane reads KS contiguous in-bounds doubles of each candidate row.
+//    The compiler schedules the refills.
+// amap entries are read two tiles ahead, norms one tile ahead.
 // The result layout, filter, survivor queue and heap updates are those of
 // knn_mfma_f64_kernel; the candidate norms and ids of a lane's result values
 // come from the lanes that loaded them, by lane shuffles.
 // ---------------------------------------------------------------------------
-template <int KS, int QB, int CB, int W>
+template <int KS, int QB, int CB, int W, bool LINE>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void knn_mfma_f64_direct_kernel(
     KnnArgs<double> a) {
+  static_assert(KS % 4 == 0, "k-steps in groups of four");
   constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
   static_assert(QB <= 2, "query block fits one bit");
   typedef int idv __attribute__((ext_vector_type(CB * 4)));
@@ -525,7 +534,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
 
   const int own0 = grp * KS, own1 = min(own0 + KS, d);  // dims this lane's group owns
-  const int st = min(own0, d - KS);                        // first dim it reads
+  const int st = LINE ? 2 * grp : min(own0, d - KS);       // first dim it reads
+  // Offset of k-step s from st in either deal.
+  auto step_dim = [](int s) -> int { return LINE ? 8 * (s >> 1) + (s & 1) : s; };
   double bq[QB][KS];
   double qn[QB];
   int ql[QB];
@@ -540,8 +551,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     qn[qb] = qv[qb] ? a.XB2[a.bmap[jj]] : 0.0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const int p = st + s;
-      bq[qb][s] = (qv[qb] && p >= own0 && p < own1) ? xq[p] : 0.0;
+      const int p = st + step_dim(s);
+      bq[qb][s] = (qv[qb] && (LINE || (p >= own0 && p < own1))) ? xq[p] : 0.0;
     }
   }
 
@@ -561,7 +572,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   auto row = [&](const Ids &ids, int cb) -> const double * {
     return a.XA + (size_t)ids.v[cb] * d + st;
   };
-  auto load_op = [&](const double *r, int s) -> double { return r[s]; };
+  auto load_op = [&](const double *r, int s) -> double { return r[step_dim(s)]; };
 
   double cn[CB];  // norms of this lane's candidates in the current tile
   int ci[CB];     // their ids (INT_MAX for empty slots)
@@ -680,7 +691,16 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
-        xb[s][cb] = load_op(nr[cb], s);
+        if (!LINE) xb[s][cb] = load_op(nr[cb], s);
+      }
+      if (LINE) {
+        if ((s & 3) == 3) {
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+            for (int u = 3; u >= 0; --u) xb[s - u][cb] = load_op(nr[cb], s - u);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
 
@@ -1124,9 +1144,9 @@ int mfma_blocks_per_cu(int ks, int cfg) {
 
 // Direct-form workgroups (one wave each) per CU that registers allow; 0 for
 // an instantiation that spills.
-template <int KS>
+template <int KS, bool LINE>
 int mfma_direct_reg_blocks() {
-  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, 2, 2, 1>;
+  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, 2, 2, 1, LINE>;
   hipFuncAttributes at;
   if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;
   int b = 0;
@@ -1134,14 +1154,19 @@ int mfma_direct_reg_blocks() {
   return b;
 }
 
-int mfma_direct_blocks_per_cu(int ks) {
-  static std::atomic<int> cache[3];
-  const int ki = ks == 4 ? 0 : ks == 8 ? 1 : 2;
+int mfma_direct_blocks_per_cu(int ks, bool line) {
+  static std::atomic<int> cache[6];
+  const int ki = (ks == 4 ? 0 : ks == 8 ? 1 : 2) + (line ? 3 : 0);
   int b = cache[ki].load(std::memory_order_relaxed);  // blocks + 1
   if (b > 0) return b - 1;
-  b = ks == 4 ? mfma_direct_reg_blocks<4>() : ks == 8 ? mfma_direct_reg_blocks<8>() : mfma_direct_reg_blocks<16>();
+  if (line)
+    b = ks == 4 ? mfma_direct_reg_blocks<4, true>() : ks == 8 ? mfma_direct_reg_blocks<8, true>()
+                                                              : mfma_direct_reg_blocks<16, true>();
+  else
+    b = ks == 4 ? mfma_direct_reg_blocks<4, false>() : ks == 8 ? mfma_direct_reg_blocks<8, false>()
+                                                               : mfma_direct_reg_blocks<16, false>();
   cache[ki].store(b + 1, std::memory_order_relaxed);
-  if (verbose()) fprintf(stderr, "gsknn: direct ks=%d: %d workgroups/CU by registers\n", ks, b);
+  if (verbose()) fprintf(stderr, "gsknn: direct ks=%d line=%d: %d workgroups/CU by registers\n", ks, (int)line, b);
   return b;
 }
 
@@ -1193,7 +1218,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     if (direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 16 && d >= ks) {
       const size_t b = mfma_direct_lds(32, k);
       if (b <= kLdsMax) {
-        const int blocks = std::min((int)(kLdsMax / b), mfma_direct_blocks_per_cu(ks));
+        const int blocks = std::min((int)(kLdsMax / b), mfma_direct_blocks_per_cu(ks, d == 4 * ks));
         if (blocks > 0 && 2 * blocks >= best) {
           best = 2 * blocks;
           p.direct = true;
@@ -1273,14 +1298,15 @@ template <int KS>
 hipError_t launch_mfma_direct_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
   const dim3 grid(p.gx, p.nsplit);
   hipError_t e = hipSuccess;
-#define GSKNN_DIRECT_CASE(W_)                                                 \
-  if (p.w == W_) {                                                            \
-    auto kern = knn_mfma_f64_direct_kernel<KS, 2, 2, W_>;                     \
+#define GSKNN_DIRECT_CASE(W_, LINE_)                                          \
+  if (p.w == W_ && (a.d == 4 * KS) == LINE_) {                                \
+    auto kern = knn_mfma_f64_direct_kernel<KS, 2, 2, W_, LINE_>;              \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
     return hipGetLastError();                                                 \
   }
-  GSKNN_DIRECT_CASE(1)
+  GSKNN_DIRECT_CASE(1, true)
+  GSKNN_DIRECT_CASE(1, false)
 #undef GSKNN_DIRECT_CASE
   return hipErrorNotSupported;
 }

@@ -573,15 +573,15 @@ def _mfma_case(X, amaps, bmap, k, mode):
 
 @pytest.mark.parametrize("form", ["direct", "tile"])
 @pytest.mark.parametrize("d,k,nq", [(64, 32, 1000), (64, 1, 700), (63, 200, 500), (50, 400, 300),
-                                    (33, 17, 37), (17, 32, 900), (5, 9, 400), (3, 8, 300),
-                                    (64, 64, 20)])
+                                    (33, 17, 37), (32, 20, 800), (17, 32, 900), (16, 16, 600),
+                                    (5, 9, 400), (3, 8, 300), (64, 64, 20)])
 def test_mfma_direct_and_tile_forms(form, d, k, nq):
     """FP64 MFMA in both forms: the direct form (each wave streams candidate
-    operands into registers; dims dealt to lane groups in runs moved back to
-    stay inside the row) and the LDS-tile kernel.  Partial last tile (3001
-    candidates), pre-filled rows, candidate splits for few queries, dims that
-    leave some lane runs overlapping (d = 5, 17, 33, 50, 63) and d = 3, below
-    the direct form's run length (falls back to the tile kernel)."""
+    operands into registers) and the LDS-tile kernel.  Partial last tile (3001
+    candidates), pre-filled rows, candidate splits for few queries; the
+    direct form's line deal (d = 16, 32, 64), its contiguous runs moved back
+    to stay inside the row (d = 5, 17, 33, 50, 63), and d = 3, below its run
+    length (falls back to the tile kernel)."""
     print(f"[gpu] mfma form={form} d={d} k={k} nq={nq}", flush=True)
     g = np.random.Generator(np.random.PCG64(d * 7919 + k * 31 + nq))
     X = g.standard_normal((6000, d))
