@@ -682,7 +682,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
     // Each operand register is refilled with tile t + 1's value once the
     // MFMAs of tile t have read it (on the last tile the refill reads valid
-    // rows and is never used).
+    // rows and is never used).  The wave issues its MFMAs at raised priority,
+    // so of the two waves of a SIMD the one in its MFMA phase keeps the
+    // matrix pipe while the other runs its epilogue.
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -700,9 +703,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
             for (int u = 3; u >= 0; --u) xb[s - u][cb] = load_op(nr[cb], s - u);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        // (None after the last k-step: the first candidate block's epilogue
+        // may start under the second block's MFMAs.)
+        if (s + 1 < KS) __builtin_amdgcn_sched_barrier(0);
       }
     }
+    __builtin_amdgcn_s_setprio(0);
 
     // Epilogue: one FMA and one compare per pair; wave masks in scalar
     // registers; per-lane bits, exact distances and ids only when a pair
