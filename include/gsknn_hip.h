@@ -27,10 +27,11 @@ extern "C" {
 
 /* Contraction selection.
  * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
- * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA
- * (for d <= 64 the form that streams candidates into registers per wave,
- * when the heaps fit its LDS); MFMA_TILE = FP64 MFMA with the LDS-tile
- * kernel only (same results within the FP64 tolerance).
+ * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA,
+ * for d <= 128 the form that streams candidates into registers per wave
+ * whenever the heaps fit its LDS (AUTO takes it only when it keeps at least
+ * as many queries resident as the LDS-tile kernel); MFMA_TILE = FP64 MFMA
+ * with the LDS-tile kernel only (same results within the FP64 tolerance).
  * FP32: AUTO/MFMA = FP32 MFMA candidate pass + re-rank in the oracle's FP32
  * arithmetic when d >= 8 and m >= 64, else the exact VALU kernel; EXACT =
  * always the VALU kernel.  Every FP32 mode returns the oracle's distance bits. */

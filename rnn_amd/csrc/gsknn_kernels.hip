@@ -1072,6 +1072,12 @@ int mfma_ks(int d) {
 // query fragments already take most of the registers.
 constexpr int mfma_cb(int ks) { return ks >= 32 ? 1 : 2; }
 
+// Direct form: query blocks and candidate blocks per wave; the query
+// fragments (QB KS doubles) and the operand registers (CB KS doubles) must
+// fit two waves per SIMD.
+constexpr int direct_qb(int ks) { return ks <= 16 ? 2 : 1; }
+constexpr int direct_cb(int ks) { return ks <= 16 ? 2 : 1; }
+
 // LDS of the direct form: the Q heap rows only.
 size_t mfma_direct_lds(int Q, int k) { return al16(sizeof(double) * (size_t)k * Q) + sizeof(int) * (size_t)k * Q; }
 
@@ -1152,7 +1158,7 @@ int mfma_blocks_per_cu(int ks, int cfg) {
 // an instantiation that spills.
 template <int KS, bool LINE>
 int mfma_direct_reg_blocks() {
-  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, 2, 2, 1, LINE>;
+  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, direct_qb(KS), direct_cb(KS), 1, LINE>;
   hipFuncAttributes at;
   if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;
   int b = 0;
@@ -1161,16 +1167,16 @@ int mfma_direct_reg_blocks() {
 }
 
 int mfma_direct_blocks_per_cu(int ks, bool line) {
-  static std::atomic<int> cache[6];
-  const int ki = (ks == 4 ? 0 : ks == 8 ? 1 : 2) + (line ? 3 : 0);
+  static std::atomic<int> cache[8];
+  const int ki = (ks == 4 ? 0 : ks == 8 ? 1 : ks == 16 ? 2 : 3) + (line ? 4 : 0);
   int b = cache[ki].load(std::memory_order_relaxed);  // blocks + 1
   if (b > 0) return b - 1;
   if (line)
     b = ks == 4 ? mfma_direct_reg_blocks<4, true>() : ks == 8 ? mfma_direct_reg_blocks<8, true>()
-                                                              : mfma_direct_reg_blocks<16, true>();
+        : ks == 16 ? mfma_direct_reg_blocks<16, true>() : mfma_direct_reg_blocks<32, true>();
   else
     b = ks == 4 ? mfma_direct_reg_blocks<4, false>() : ks == 8 ? mfma_direct_reg_blocks<8, false>()
-                                                               : mfma_direct_reg_blocks<16, false>();
+        : ks == 16 ? mfma_direct_reg_blocks<16, false>() : mfma_direct_reg_blocks<32, false>();
   cache[ki].store(b + 1, std::memory_order_relaxed);
   if (verbose()) fprintf(stderr, "gsknn: direct ks=%d line=%d: %d workgroups/CU by registers\n", ks, (int)line, b);
   return b;
@@ -1193,7 +1199,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       const char *e = getenv("GSKNN_MFMA_CFG");
       return e ? atoi(e) : -1;
     }();
-    int best = 0;
+    int best = 0, best_waves = 0;
     for (int c = 0; c < kNumMfmaCfg; ++c) {
       if (pinned >= 0 && pinned < kNumMfmaCfg && c != pinned) continue;
       const int qb = kMfmaCfg[c][0], w = kMfmaCfg[c][1], nb = kMfmaCfg[c][2];
@@ -1205,6 +1211,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       const int score = blocks * w * qb;
       if (score > best) {
         best = score;
+        best_waves = blocks * w;
         p.qb = qb;
         p.w = w;
         p.nb = nb;
@@ -1213,27 +1220,33 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       }
     }
     tile_c = 16 * mfma_cb(ks);
-    // The direct form (d <= 64): one wave of 32 queries per workgroup, only
-    // the heap rows in LDS.  Taken when its score (resident waves x QB = 2)
-    // is at least the LDS-tile kernel's.  GSKNN_MFMA_DIRECT=0 turns it off
+    // The direct form (d <= 128): one wave of 16 QB queries per workgroup
+    // (QB = 2 up to d = 64, 1 above), only the heap rows in LDS.  Taken by
+    // MODE_MFMA wherever it fits; by MODE_AUTO when it keeps at least as many
+    // waves resident per CU as the LDS-tile kernel's shape (measured faster
+    // at equal residency for d = 64, 96 and 128).
+    // GSKNN_MFMA_DIRECT=0 turns it off, =2 takes it wherever it fits
     // (tuning).
-    static const bool direct_on = [] {
+    static const int direct_env = [] {
       const char *e = getenv("GSKNN_MFMA_DIRECT");
-      return !(e && *e == '0');
+      return e ? atoi(e) : 1;
     }();
-    if (direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 16 && d >= ks) {
-      const size_t b = mfma_direct_lds(32, k);
+    const bool direct_on = direct_env != 0;
+    if (direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 32 && d >= ks) {
+      const int qb = direct_qb(ks), Q = 16 * qb;
+      const size_t b = mfma_direct_lds(Q, k);
       if (b <= kLdsMax) {
         const int blocks = std::min((int)(kLdsMax / b), mfma_direct_blocks_per_cu(ks, d == 4 * ks));
-        if (blocks > 0 && 2 * blocks >= best) {
-          best = 2 * blocks;
+        if (blocks > 0 && (blocks >= best_waves || direct_env == 2 || mode == MODE_MFMA)) {
+          best = qb * blocks;
           p.direct = true;
-          p.qb = 2;
+          p.qb = qb;
           p.w = 1;
           p.nb = 0;
-          p.qpb = 32;
+          p.qpb = Q;
           p.lds = b;
           p.per_cu = blocks;
+          tile_c = 16 * direct_cb(ks);
         }
       }
     }
@@ -1304,9 +1317,10 @@ template <int KS>
 hipError_t launch_mfma_direct_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
   const dim3 grid(p.gx, p.nsplit);
   hipError_t e = hipSuccess;
+  constexpr int QB = direct_qb(KS), CB = direct_cb(KS);
 #define GSKNN_DIRECT_CASE(W_, LINE_)                                          \
   if (p.w == W_ && (a.d == 4 * KS) == LINE_) {                                \
-    auto kern = knn_mfma_f64_direct_kernel<KS, 2, 2, W_, LINE_>;              \
+    auto kern = knn_mfma_f64_direct_kernel<KS, QB, CB, W_, LINE_>;            \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
     return hipGetLastError();                                                 \
@@ -1455,6 +1469,7 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
         case 4: e = launch_mfma_direct_ks<4>(p, a, s); break;
         case 8: e = launch_mfma_direct_ks<8>(p, a, s); break;
         case 16: e = launch_mfma_direct_ks<16>(p, a, s); break;
+        case 32: e = launch_mfma_direct_ks<32>(p, a, s); break;
         default: return hipErrorNotSupported;
       }
     } else {

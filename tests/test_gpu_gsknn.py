@@ -530,8 +530,9 @@ def test_split_merge_paths_exact(k, nq):
 
 @pytest.mark.parametrize("d,k", [(64, 500), (40, 420), (100, 700)])
 def test_mfma_large_k_single_wave_shape(d, k):
-    """FP64 MFMA with heaps so large that only one wave of 16 queries fits a
-    workgroup's LDS (fewer waves than candidate blocks per tile)."""
+    """FP64 MFMA, LDS-tile kernel, with heaps so large that only one wave of
+    16 queries fits a workgroup's LDS (fewer waves than candidate blocks per
+    tile)."""
     import torch
     print(f"[gpu] mfma large k d={d} k={k}", flush=True)
     dev = torch.device("cuda")
@@ -544,7 +545,7 @@ def test_mfma_large_k_single_wave_shape(d, k):
     D = torch.full((200, k), float("inf"), dtype=torch.float64, device=dev)
     I = torch.full((200, k), -1, dtype=torch.int32, device=dev)
     gsknn.dknn_dev(tX, tX2, torch.from_numpy(amap).to(dev), tX, tX2,
-                   torch.from_numpy(bmap).to(dev), D, I, mode=gsknn.MODE_MFMA)
+                   torch.from_numpy(bmap).to(dev), D, I, mode=gsknn.MODE_MFMA_TILE)
     torch.cuda.synchronize()
     Dr, Ir = ds.empty_heap(200, k)
     oracle.gsknn(X, X2, amap, X, X2, bmap, Dr, Ir, canonical=True)
@@ -574,13 +575,15 @@ def _mfma_case(X, amaps, bmap, k, mode):
 @pytest.mark.parametrize("form", ["direct", "tile"])
 @pytest.mark.parametrize("d,k,nq", [(64, 32, 1000), (64, 1, 700), (63, 200, 500), (50, 400, 300),
                                     (33, 17, 37), (32, 20, 800), (17, 32, 900), (16, 16, 600),
-                                    (5, 9, 400), (3, 8, 300), (64, 64, 20)])
+                                    (5, 9, 400), (3, 8, 300), (64, 64, 20), (128, 32, 700),
+                                    (100, 24, 500), (65, 40, 60)])
 def test_mfma_direct_and_tile_forms(form, d, k, nq):
     """FP64 MFMA in both forms: the direct form (each wave streams candidate
     operands into registers) and the LDS-tile kernel.  Partial last tile (3001
     candidates), pre-filled rows, candidate splits for few queries; the
-    direct form's line deal (d = 16, 32, 64), its contiguous runs moved back
-    to stay inside the row (d = 5, 17, 33, 50, 63), and d = 3, below its run
+    direct form's line deal (d = 16, 32, 64, and 128 with one query block and
+    one candidate block per wave), its contiguous runs moved back to stay
+    inside the row (d = 5, 17, 33, 50, 63, 65, 100), and d = 3, below its run
     length (falls back to the tile kernel)."""
     print(f"[gpu] mfma form={form} d={d} k={k} nq={nq}", flush=True)
     g = np.random.Generator(np.random.PCG64(d * 7919 + k * 31 + nq))
