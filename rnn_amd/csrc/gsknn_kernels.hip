@@ -788,6 +788,171 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 }
 
 // ---------------------------------------------------------------------------
+// Small candidate sets (m <= 64 P, P <= 32, k <= 64, d <= 64): one wave per
+// query, four queries per workgroup, the oracle's arithmetic.  The workgroup
+// stages the candidates in chunks of 64 CH rows into LDS, dim-major (row
+// loads coalesced along each row; lane l then reads candidate l + 64 u of
+// the chunk at dim p without bank conflicts).  Lane l computes the exact
+// distances of candidates l, l + 64, ... (dot product in p order, product
+// and sum rounded separately, then x(-2), + XA2[amap[i]], + XB2[bmap[j]]) and
+// keeps them in registers beside entry l of the resident row.  The wave then
+// takes the k smallest entries of (resident row + candidates) under the
+// canonical (distance, index) order by k rounds of a wave arg-min and writes
+// them descending.  That is the row the heap insertion rule yields (a
+// candidate enters only when it is below the current k-th, so the row always
+// holds the k smallest entries seen, duplicates included), with no heap, no
+// candidate split and no merge.
+// ---------------------------------------------------------------------------
+constexpr int kSmallMaxK = 64;
+constexpr int kSmallMaxP = 32;
+
+// Candidates per lane per staged chunk: LDS holds the [d][64 CH] tile
+// (CH d <= 128: at most 64 KiB for FP64) plus the four query rows.
+constexpr int small_ch(int d) { return d <= 16 ? 8 : d <= 32 ? 4 : 2; }
+
+template <typename T, int P, int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void knn_small_m_kernel(KnnArgs<T> a) {
+  static_assert(P <= 32 && P % CH == 0, "candidate slots per lane");
+  constexpr int C = 64 * CH;  // candidates per chunk
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *xs = (T *)smem;           // [d][C]
+  T *qs = xs + (size_t)C * a.d;  // [4][64]: the query rows
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int j = blockIdx.x * 4 + w;
+  const bool qok = j < a.n;  // waves past n still help stage
+  const int k = a.k, d = a.d, m = a.m;
+  const int qid = qok ? a.bmap[j] : 0;
+  if (lane < d) qs[w * 64 + lane] = a.XB[(size_t)qid * d + lane];
+
+  int id[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const int c = lane + 64 * i;
+    id[i] = c < m ? a.amap[c] : -1;
+  }
+  T v[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) v[i] = T(0);
+#pragma unroll
+  for (int ch = 0; ch < P / CH; ++ch) {
+    const int c0 = ch * C;
+    if (c0 < m) {  // uniform
+      // Stage rows c0 .. c0 + C - 1 (rows past m read row 0, never used):
+      // 16 threads per row, thread t reading dims t % 16, t % 16 + 16, ...,
+      // 16 rows per pass and eight passes in flight at a time.
+      const int p0 = tid & 15, rr = tid >> 4;
+#pragma unroll
+      for (int g = 0; g < C / 128; ++g) {
+        int rid[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = c0 + (g * 8 + u) * 16 + rr;
+          rid[u] = c < m ? a.amap[c] : 0;
+        }
+#pragma unroll
+        for (int h = 0; h < 64; h += 16) {
+          if (p0 + h < d) {
+            T x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = a.XA[(size_t)rid[u] * d + p0 + h];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xs[(p0 + h) * C + (g * 8 + u) * 16 + rr] = x[u];
+          }
+        }
+      }
+      __syncthreads();
+      if (qok) {
+        for (int p = 0; p < d; ++p) {
+          const T qp = qs[w * 64 + p];
+#pragma unroll
+          for (int u = 0; u < CH; ++u)
+            v[ch * CH + u] = xadd(v[ch * CH + u], xmul(xs[p * C + lane + 64 * u], qp));
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (!qok) return;
+  const T qn = a.XB2[qid];
+  // Taken entries and empty slots: bit i of tk for candidate slot i, rtk for
+  // the resident entry.
+  unsigned tk = 0u;
+  bool rtk = false;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    if (lane + 64 * i < m)
+      v[i] = xadd(xadd(xmul(v[i], T(-2)), a.XA2[id[i]]), qn);
+    else
+      tk |= 1u << i;
+  }
+  const size_t rbase = (size_t)heap_row(a, j) * a.ldk;
+  T rv = T(0);
+  int ri = -1;
+  if (lane < k) {
+    rv = a.D[rbase + lane];
+    ri = a.I[rbase + lane];
+  } else {
+    rtk = true;
+  }
+
+  // This lane's smallest entry not yet taken: (bd, bi) at slot bs (P for the
+  // resident entry, -1 when none is left).
+  T bd = T(0);
+  int bi = 0, bs = -1;
+  auto rescan = [&]() {
+    bs = -1;
+    if (!rtk) {
+      bd = rv;
+      bi = ri;
+      bs = P;
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      if (!((tk >> i) & 1u) && (bs < 0 || cless(v[i], id[i], bd, bi))) {
+        bd = v[i];
+        bi = id[i];
+        bs = i;
+      }
+  };
+  rescan();
+  T outd = T(0);
+  int outi = -1;
+  for (int r = 0; r < k; ++r) {
+    // Wave arg-min of (bd, bi, lane); lanes with nothing left never win.
+    T md = bd;
+    int mi = bi, ml = bs < 0 ? 64 : lane;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const T od = __shfl_xor(md, o);
+      const int oi = __shfl_xor(mi, o);
+      const int ol = __shfl_xor(ml, o);
+      const bool take = ol < 64 && (ml == 64 || cless(od, oi, md, mi) ||
+                                    (od == md && oi == mi && ol < ml));
+      if (take) {
+        md = od;
+        mi = oi;
+        ml = ol;
+      }
+    }
+    if (lane == k - 1 - r) {  // rows are written descending
+      outd = md;
+      outi = mi;
+    }
+    if (lane == ml) {
+      if (bs == P)
+        rtk = true;
+      else
+        tk |= 1u << bs;
+      rescan();
+    }
+  }
+  if (lane < k) {
+    a.D[rbase + lane] = outd;
+    a.I[rbase + lane] = outi;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Merge of split partial lists into the resident rows: one thread per query.
 // Each partial list is ascending, so the scan of a list stops at its first
 // entry that does not beat the root.  Sentinel entries (copies of the
@@ -1051,6 +1216,7 @@ struct Plan {
   int ks = 0, qb = 0, w = 0, nb = 1;  // mfma
   bool direct = false;                // mfma: the direct form (no LDS tile)
   int per_cu = 0;                     // direct form: resident workgroups per CU
+  int small = 0;                      // exact, one wave per query: candidate slots per lane
   size_t lds = 0;
   int gx = 0, nsplit = 1, cps = 0;
 };
@@ -1307,6 +1473,24 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     p.nsplit = 1;
     p.cps = m;
   }
+  // Exact calls too small to fill the chip with one thread per query
+  // (they would split the candidates and merge) and with m <= 2048, k <= 64,
+  // d <= 64:
+  // one wave per query instead (knn_small_m_kernel), no split.
+  // GSKNN_SMALL_M=0 turns it off (tuning).
+  static const bool small_on = [] {
+    const char *e = getenv("GSKNN_SMALL_M");
+    return !(e && *e == '0');
+  }();
+  if (small_on && p.kind == 0 && p.nsplit > 1 && m <= 64 * kSmallMaxP && k <= kSmallMaxK &&
+      d <= 64) {
+    p.small = m <= 512 ? 8 : m <= 1024 ? 16 : 32;
+    p.qpb = 4;
+    p.gx = (n + 3) / 4;
+    p.lds = 0;
+    p.nsplit = 1;
+    p.cps = m;
+  }
   return p;
 }
 
@@ -1357,6 +1541,31 @@ hipError_t launch_mfma_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s
   GSKNN_MFMA_CASE(1, 1, 1)
 #undef GSKNN_MFMA_CASE
   return hipErrorNotSupported;
+}
+
+template <typename T>
+hipError_t launch_small_m(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
+  const dim3 grid(p.gx);
+  void (*kern)(KnnArgs<T>) = nullptr;
+  const int ch = small_ch(a.d);
+#define GSKNN_SMALL_CASE(P_, CH_) \
+  if (p.small == P_ && ch == CH_) kern = knn_small_m_kernel<T, P_, CH_>;
+  GSKNN_SMALL_CASE(8, 8)
+  GSKNN_SMALL_CASE(8, 4)
+  GSKNN_SMALL_CASE(8, 2)
+  GSKNN_SMALL_CASE(16, 8)
+  GSKNN_SMALL_CASE(16, 4)
+  GSKNN_SMALL_CASE(16, 2)
+  GSKNN_SMALL_CASE(32, 8)
+  GSKNN_SMALL_CASE(32, 4)
+  GSKNN_SMALL_CASE(32, 2)
+#undef GSKNN_SMALL_CASE
+  if (kern == nullptr) return hipErrorNotSupported;
+  const size_t lds = ((size_t)64 * ch * a.d + 256) * sizeof(T);  // [d][C] tile + 4 query rows
+  hipError_t e = set_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -1489,7 +1698,7 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
       return hipErrorNotSupported;
     }
   } else {
-    e = launch_exact(p, a, s);
+    e = p.small ? launch_small_m(p, a, s) : launch_exact(p, a, s);
   }
   if (e != hipSuccess) return e;
   if (p.nsplit > 1) e = launch_merge(a, s);
