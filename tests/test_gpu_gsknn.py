@@ -609,3 +609,57 @@ def test_mfma_forms_lattice_ties(form):
     mode = gsknn.MODE_MFMA if form == "direct" else gsknn.MODE_MFMA_TILE
     Dg, Ig, Dr, Ir, _ = _mfma_case(X, (amap,), bmap, 24, mode)
     assert_rows(Dg, Ig, Dr, Ir)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("d,m,k,nq", [(16, 700, 16, 300), (7, 2048, 64, 100), (64, 1500, 33, 50),
+                                      (33, 130, 5, 900), (1, 1999, 40, 64)])
+def test_small_m_one_wave_per_query(dtype, d, m, k, nq):
+    """Exact calls too small to fill the chip (few queries, m <= 2048,
+    k <= 64): one wave per query, candidates staged through LDS, k rounds of
+    a wave arg-min.  Bit-identical to the oracle on carried-over rows (a first
+    call, then a second over other candidates) written through a heap-row
+    map; lane slots left partly empty (m not a multiple of 64); d from 1 to
+    64 (chunks of 512, 256 and 128 rows)."""
+    import torch
+    print(f"[gpu] small-m {np.dtype(dtype).name} d={d} m={m} k={k} nq={nq}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(d * 131 + m + k))
+    X = g.standard_normal((4000, d)).astype(dtype)
+    X2 = oracle.sqnorm(X).astype(dtype)
+    first = g.integers(0, 4000, 300).astype(np.int32)
+    amap = g.integers(0, 4000, m).astype(np.int32)
+    bmap = g.integers(0, 4000, nq).astype(np.int32)
+    hrow = g.permutation(nq + 17)[:nq].astype(np.int32)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((nq + 17, k), float("inf"), dtype=torch.from_numpy(X).dtype, device=dev)
+    I = torch.full((nq + 17, k), -1, dtype=torch.int32, device=dev)
+    th = torch.from_numpy(hrow).to(dev)
+    run = gsknn.dknn_dev if dtype == np.float64 else gsknn.sknn_dev
+    for am in (first, amap):
+        run(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2, torch.from_numpy(bmap).to(dev), D, I,
+            hrow=th, mode=gsknn.MODE_EXACT)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(nq, k, dtype)
+    for am in (first, amap):
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(D.cpu().numpy()[hrow], I.cpu().numpy()[hrow], Dr, Ir)
+    untouched = np.setdiff1d(np.arange(nq + 17), hrow)
+    assert np.all(I.cpu().numpy()[untouched] == -1)
+
+
+def test_small_m_lattice_ties():
+    """Integer lattice (exact ties) through the one-wave-per-query kernel:
+    the lower index wins, bit-identical distances, duplicates kept."""
+    print("[gpu] small-m lattice", flush=True)
+    pts = np.array([(x, y, z) for x in range(9) for y in range(9) for z in range(9)],
+                   dtype=np.float64)
+    X2 = oracle.sqnorm(pts)
+    g = np.random.Generator(np.random.PCG64(77))
+    amap = np.concatenate([g.permutation(len(pts))[:900], g.integers(0, len(pts), 300)]).astype(np.int32)
+    bmap = np.arange(0, len(pts), 3, dtype=np.int32)
+    D, I = ds.empty_heap(bmap.size, 24)
+    Dr, Ir = D.copy(), I.copy()
+    gsknn.dgsknn_heap(pts, X2, amap, pts, X2, bmap, D, I)
+    oracle.gsknn(pts, X2, amap, pts, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(D, I, Dr, Ir)
