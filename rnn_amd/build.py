@@ -12,6 +12,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -38,17 +39,26 @@ def sources():
 
 
 def build(verbose: bool = False) -> str:
+    """Compile every source (in parallel, at most MAX_JOBS or 8 at once) and
+    link the library; raises CalledProcessError on the first failure."""
     os.makedirs(LIBDIR, exist_ok=True)
     hipcc = os.path.join(ROCM, "bin", "hipcc")
-    objs = []
+    cmds, objs = [], []
     for src in sources():
         obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [hipcc] + HIPFLAGS + lang + ["-c", src, "-o", obj]
+        cmds.append([hipcc] + HIPFLAGS + lang + ["-c", src, "-o", obj])
+        objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     if verbose:
