@@ -72,23 +72,35 @@ __device__ __forceinline__ void group_reg(sptr<int> am, sptr<T> xs, int c0, int 
                                           const T (&qreg)[D], bool valid,
                                           LaneQueue<T, kLaneQueue> &Q, T *hd, int *hi, int q,
                                           int k, T &rd, int &ri) {
-  constexpr int P = l1_chunk<T>() < D ? l1_chunk<T>() : D;
+  // Half chunks (32 bytes of each candidate row) in two buffers: the loads
+  // of chunk c + 2 wait for chunk c's sums (issued all at once they would
+  // need TC * D * sizeof(T) / 4 SGPRs) and are in flight while chunk c + 1
+  // is summed.
+  constexpr int P0 = l1_chunk<T>() < D ? l1_chunk<T>() : D;
+  constexpr int P = P0 / 2;
+  constexpr int NC = D / P;
   int id[TC];
   group_ids<TC, FULL>(am, c0, cend, id);
   T tot[TC];
 #pragma unroll
   for (int u = 0; u < TC; ++u) tot[u] = T(0);
+  T cv[2][TC][P];
 #pragma unroll
-  for (int p0 = 0; p0 < D; p0 += P) {
-    // Each chunk's scalar loads wait for the previous chunk's sums: issued
-    // all at once they would need TC * D * sizeof(T) / 4 SGPRs.
-    const sptr<T> base = p0 == 0 ? xs : after<T, TC>(xs, tot);
-    T cv[TC][P];
+  for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int u = 0; u < TC; ++u)
 #pragma unroll
-      for (int i = 0; i < P; ++i) cv[u][i] = base[(size_t)id[u] * D + p0 + i];
-    chunk_sum<T, TC, P>(qreg + p0, cv, tot);
+      for (int i = 0; i < P; ++i) cv[c][u][i] = xs[(size_t)id[u] * D + c * P + i];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    chunk_sum<T, TC, P>(qreg + c * P, cv[c & 1], tot);
+    if (c + 2 < NC) {
+      const sptr<T> base = after<T, TC>(xs, tot);
+#pragma unroll
+      for (int u = 0; u < TC; ++u)
+#pragma unroll
+        for (int i = 0; i < P; ++i) cv[c & 1][u][i] = base[(size_t)id[u] * D + (c + 2) * P + i];
+    }
   }
   queue_offer<T, kLaneQueue, TC>(Q, tot, id, FULL ? TC : cend - c0, valid, hd, hi, QPB, q, k,
                                  rd, ri);
