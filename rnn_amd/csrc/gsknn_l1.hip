@@ -30,7 +30,8 @@ constexpr int kL1Block = 256;  // DKNN_KC: dimensions per summation block
 constexpr int kL1TC = 4;       // candidates per group
 
 // Dimensions per scalar chunk: 64 bytes of each candidate row (one
-// s_load_dwordx16 per candidate).
+// s_load_dwordx16 per candidate) in the LDS-query path; the register path
+// reads half chunks, two buffers deep.
 template <typename T>
 constexpr int l1_chunk() {
   return 64 / (int)sizeof(T);
