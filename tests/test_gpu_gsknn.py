@@ -663,3 +663,40 @@ def test_small_m_lattice_ties():
     gsknn.dgsknn_heap(pts, X2, amap, pts, X2, bmap, D, I)
     oracle.gsknn(pts, X2, amap, pts, X2, bmap, Dr, Ir, canonical=True)
     assert_rows(D, I, Dr, Ir)
+
+
+@pytest.mark.parametrize("form", ["direct", "tile"])
+@pytest.mark.parametrize("d,nq", [(64, 300), (48, 2500)])
+def test_mfma_forms_heap_row_map(form, d, nq):
+    """FP64 MFMA in both forms writing through a heap-row map into a larger
+    pool (rows outside the map untouched), with a pre-filling call, at a
+    size that splits the candidates (300 queries) and one that does not."""
+    import torch
+    print(f"[gpu] mfma hrow form={form} d={d} nq={nq}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(d * 17 + nq))
+    X = g.standard_normal((8000, d))
+    X2 = oracle.sqnorm(X)
+    first = g.integers(0, 8000, 700).astype(np.int32)
+    amap = g.integers(0, 8000, 6001).astype(np.int32)
+    bmap = g.integers(0, 8000, nq).astype(np.int32)
+    pool = nq + 123
+    hrow = g.permutation(pool)[:nq].astype(np.int32)
+    k = 20
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((pool, k), float("inf"), dtype=torch.float64, device=dev)
+    I = torch.full((pool, k), -1, dtype=torch.int32, device=dev)
+    mode = gsknn.MODE_MFMA if form == "direct" else gsknn.MODE_MFMA_TILE
+    for am in (first, amap):
+        gsknn.dknn_dev(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2,
+                       torch.from_numpy(bmap).to(dev), D, I,
+                       hrow=torch.from_numpy(hrow).to(dev), mode=mode)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(nq, k)
+    for am in (first, amap):
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    Dg, Ig = D.cpu().numpy(), I.cpu().numpy()
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg[hrow], Ig[hrow], Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+    untouched = np.setdiff1d(np.arange(pool), hrow)
+    assert np.all(Ig[untouched] == -1)
