@@ -36,9 +36,13 @@ def assert_rows(D, I, Dref, Iref, X2q=None, X2c_rows=None, tol=None):
         assert np.array_equal(Ds.view(np.uint8), Dr.view(np.uint8)), \
             f"max |dD| = {np.nanmax(np.abs(Ds - Dr))}"
     else:
-        scale = np.maximum(np.abs(Dr), X2q[:, None] + X2c_rows)
-        bad = np.abs(Ds - Dr) > tol * scale
-        assert not bad.any(), f"{bad.sum()} distances beyond tolerance"
+        # Equal entries (placeholders at +inf included) pass; any other pair
+        # must be finite and within the norm-scaled tolerance.
+        same = Ds == Dr
+        with np.errstate(invalid="ignore"):
+            scale = np.where(np.isfinite(Dr), np.maximum(np.abs(Dr), X2q[:, None] + X2c_rows), 0.0)
+            ok = same | (np.isfinite(Ds) & np.isfinite(Dr) & (np.abs(Ds - Dr) <= tol * scale))
+        assert ok.all(), f"{(~ok).sum()} distances beyond tolerance"
 
 
 def run_gpu(X, X2, amaps, bmap, D, I):
@@ -700,3 +704,21 @@ def test_mfma_forms_heap_row_map(form, d, nq):
     assert_rows(Dg[hrow], Ig[hrow], Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
     untouched = np.setdiff1d(np.arange(pool), hrow)
     assert np.all(Ig[untouched] == -1)
+
+
+@pytest.mark.parametrize("form", ["direct", "tile"])
+@pytest.mark.parametrize("m,k", [(5, 12), (31, 31), (33, 40)])
+def test_mfma_forms_tiny_candidate_sets(form, m, k):
+    """Fewer candidates than one tile and than k: partly empty tile slots
+    never enter a row, and rows keep (inf, -1) placeholders beyond m."""
+    print(f"[gpu] mfma tiny form={form} m={m} k={k}", flush=True)
+    g = np.random.Generator(np.random.PCG64(m * 100 + k))
+    X = g.standard_normal((500, 64))
+    amap = g.permutation(500)[:m].astype(np.int32)
+    bmap = g.integers(0, 500, 70).astype(np.int32)
+    mode = gsknn.MODE_MFMA if form == "direct" else gsknn.MODE_MFMA_TILE
+    Dg, Ig, Dr, Ir, X2 = _mfma_case(X, (amap,), bmap, k, mode)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=np.where(Is >= 0, X2[np.maximum(Is, 0)], 0.0),
+                tol=TOL64)
+    assert np.all((Ig >= 0).sum(axis=1) == min(m, k))
