@@ -638,6 +638,10 @@ def main():
                      "frac": achieved / FP64_MATRIX_PEAK, "traffic": measured_traffic(),
                      "kernel_ms": kern_ms, "flop_per_launch": flops},
         "spot_check": parity,
+        # The reference driver's GFLOPS convention, (2d + 3) flops per pair
+        # (test/test_dgsknn.c:243), over the same kernel time.
+        "flops_2d_plus_3": {"tflops": (2.0 * p.d + 3.0) * pairs / (kern_ms * 1e-3) / 1e12,
+                            "flop_per_pair": 2 * p.d + 3},
         "pcie_inclusive": {"ms": host_ms, "pairs_per_s": pairs / (host_ms * 1e-3),
                            "rows_equal_device": host_equal},
         "rows_ascending": {"ms": sort_ms, "rows": p.n, "k": p.k, "order_ok": sort_ok},
