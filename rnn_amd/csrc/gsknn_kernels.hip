@@ -806,6 +806,23 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 constexpr int kSmallMaxK = 64;
 constexpr int kSmallMaxP = 32;
 
+// Value of the lane that DPP control CTRL selects within this lane's row
+// (every lane of the wave active; CTRL is a full permutation of each row).
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL>((int)(b & 0xffffffffll)), hi = dpp_move<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Candidates per lane per staged chunk: LDS holds the [d][64 CH] tile
 // (CH d <= 128: at most 64 KiB for FP64) plus the four query rows.
 constexpr int small_ch(int d) { return d <= 16 ? 8 : d <= 32 ? 4 : 2; }
@@ -919,13 +936,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   int outi = -1;
   for (int r = 0; r < k; ++r) {
     // Wave arg-min of (bd, bi, lane); lanes with nothing left never win.
+    // Within each row of 16 lanes by DPP moves (partner lane l ^ 1, l ^ 2,
+    // then the mirror in each half row and in the row: each step pairs two
+    // groups that already hold their own minimum), across rows by lane
+    // shuffles (l ^ 16, l ^ 32).
     T md = bd;
     int mi = bi, ml = bs < 0 ? 64 : lane;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const T od = __shfl_xor(md, o);
-      const int oi = __shfl_xor(mi, o);
-      const int ol = __shfl_xor(ml, o);
+    auto combine = [&](T od, int oi, int ol) {
       const bool take = ol < 64 && (ml == 64 || cless(od, oi, md, mi) ||
                                     (od == md && oi == mi && ol < ml));
       if (take) {
@@ -933,6 +950,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         mi = oi;
         ml = ol;
       }
+    };
+    combine(dpp_move<0xB1>(md), dpp_move<0xB1>(mi), dpp_move<0xB1>(ml));     // quad_perm 1,0,3,2
+    combine(dpp_move<0x4E>(md), dpp_move<0x4E>(mi), dpp_move<0x4E>(ml));     // quad_perm 2,3,0,1
+    combine(dpp_move<0x141>(md), dpp_move<0x141>(mi), dpp_move<0x141>(ml));  // row_half_mirror
+    combine(dpp_move<0x140>(md), dpp_move<0x140>(mi), dpp_move<0x140>(ml));  // row_mirror
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      const T od = __shfl_xor(md, o);
+      const int oi = __shfl_xor(mi, o);
+      const int ol = __shfl_xor(ml, o);
+      combine(od, oi, ol);
     }
     if (lane == k - 1 - r) {  // rows are written descending
       outd = md;
