@@ -827,6 +827,21 @@ __device__ __forceinline__ double dpp_move(double v) {
 // (CH d <= 128: at most 64 KiB for FP64) plus the four query rows.
 constexpr int small_ch(int d) { return d <= 16 ? 8 : d <= 32 ? 4 : 2; }
 
+// Offer entry (x, xi) at slot xs to a lane's two smallest entries so far,
+// (bd, bi, bs) <= (sd, si, ss) in canonical order; a slot of -1 is empty.
+template <typename T>
+__device__ __forceinline__ void top2_consider(T x, int xi, int xs, T &bd, int &bi, int &bs, T &sd,
+                                              int &si, int &ss) {
+  const bool first = bs < 0 || cless(x, xi, bd, bi);
+  const bool second = !first && (ss < 0 || cless(x, xi, sd, si));
+  sd = first ? bd : (second ? x : sd);
+  si = first ? bi : (second ? xi : si);
+  ss = first ? bs : (second ? xs : ss);
+  bd = first ? x : bd;
+  bi = first ? xi : bi;
+  bs = first ? xs : bs;
+}
+
 template <typename T, int P, int CH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void knn_small_m_kernel(KnnArgs<T> a) {
   static_assert(P <= 32 && P % CH == 0, "candidate slots per lane");
@@ -913,23 +928,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
 
   // This lane's smallest entry not yet taken: (bd, bi) at slot bs (P for the
-  // resident entry, -1 when none is left).
-  T bd = T(0);
-  int bi = 0, bs = -1;
-  auto rescan = [&]() {
+  // resident entry, -1 when none is left), and the next one (sd, si) at
+  // slot ss (-1 when none, -2 when not known).  A lane wins rarely (k
+  // rounds over 64 lanes), so a win promotes the next entry and only a
+  // second win rescans the lane's entries.
+  T bd = T(0), sd = T(0);
+  int bi = 0, bs = -1, si = 0, ss = -1;
+  auto rescan = [&]() __attribute__((always_inline)) {
     bs = -1;
-    if (!rtk) {
-      bd = rv;
-      bi = ri;
-      bs = P;
-    }
+    ss = -1;
+    if (!rtk) top2_consider(rv, ri, P, bd, bi, bs, sd, si, ss);
 #pragma unroll
     for (int i = 0; i < P; ++i)
-      if (!((tk >> i) & 1u) && (bs < 0 || cless(v[i], id[i], bd, bi))) {
-        bd = v[i];
-        bi = id[i];
-        bs = i;
-      }
+      if (!((tk >> i) & 1u)) top2_consider(v[i], id[i], i, bd, bi, bs, sd, si, ss);
   };
   rescan();
   T outd = T(0);
@@ -971,7 +982,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         rtk = true;
       else
         tk |= 1u << bs;
-      rescan();
+      if (ss != -2) {  // the next entry is known (or there is none)
+        bd = sd;
+        bi = si;
+        bs = ss;
+        ss = -2;
+      } else {
+        rescan();
+      }
     }
   }
   if (lane < k) {
