@@ -187,7 +187,7 @@ def _timed(torch, stream, fn, reps):
 
 
 def precision_form(torch, dev, args):
-    """Config 3: FP32 points x = zW + 0.01 eps (latent 10), d = 256,
+    """Precision-form workload (`--workload c3`): FP32 points x = zW + 0.01 eps (latent 10), d = 256,
     m = n = 131072, k = 64.  Times the BF16-candidate + FP64 re-rank form,
     the plain FP64 MFMA path (points upcast) and the plain FP32 path, each on
     empty heaps, and compares neighbour indices and distances."""
@@ -289,7 +289,7 @@ def precision_form(torch, dev, args):
 
 
 def allnn_sweep(torch, dev, args):
-    """Config 4: N = 2^20 points, d = 32, FP32, 256-cluster mixture; 8
+    """All-NN tree-sweep workload (`--workload c4`): N = 2^20 points, d = 32, FP32, 256-cluster mixture; 8
     randomized-projection-tree iterations with leaves of 256-4096 points, k = 32,
     global heaps carried across iterations.  One persistent launch per
     iteration (graph-replayed after the first), autotuned launch shape."""
@@ -321,7 +321,7 @@ def allnn_sweep(torch, dev, args):
 
 
 def serving(torch, dev, args):
-    """Config 5: device-resident reference set of 2,000,000 x 128 FP32 points
+    """Serving workload (`--workload c5`): device-resident reference set of 2,000,000 x 128 FP32 points
     (low-rank generator, latent 16); Poisson arrivals at 200 req/s; request
     sizes log-uniform in [1, 8192] queries; k uniform in {10, 32, 100}.
     Continuous batching; p50/p99 of (completion - arrival) and pairs/s for the
@@ -368,7 +368,7 @@ def serving(torch, dev, args):
 
 
 def c1_small(torch, dev, reps=200):
-    """Config 1: N = 8192, d = 16, U[0,1), m = n = 2048, k = 16, +inf heaps;
+    """Small drop-in workload (`--workload c1`): N = 8192, d = 16, U[0,1), m = n = 2048, k = 16, +inf heaps;
     the call pair (pi[0:2048] then the disjoint candidates pi[2048:4096] merged
     into the same rows).  Latency-bound (everything fits in cache): reports us
     per call, pairs/s and the compulsory bytes per call
