@@ -1107,33 +1107,52 @@ __global__ __launch_bounds__(256) void knn_merge_wave_kernel(KnnArgs<T> a) {
     live[r] = e < k || (e < E && cless(ed[r], ei[r], md, mi));
   }
   // k rounds: each lane offers its smallest live entry; the wave minimum by
-  // (distance, index, lane) is written and retired.
-  for (int t = 0; t < k; ++t) {
-    T ld = (T)__builtin_inf();
-    int li = INT_MAX, lr = -1;
+  // (distance, index, lane) is written and retired.  Each lane keeps its two
+  // smallest live entries (slot -1: none, -2: not known), so a win promotes
+  // the second and only a lane's second win rescans its entries; the
+  // arg-min steps within a 16-lane row are DPP moves, across rows shuffles.
+  T ld = T(0), nd = T(0);
+  int li = 0, lr = -1, ni = 0, nr = -1;
+  auto rescan = [&]() __attribute__((always_inline)) {
+    lr = -1;
+    nr = -1;
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (live[r] && (lr < 0 || cless(ed[r], ei[r], ld, li))) {
-        ld = ed[r];
-        li = ei[r];
-        lr = r;
-      }
+      if (live[r]) top2_consider(ed[r], ei[r], r, ld, li, lr, nd, ni, nr);
+  };
+  rescan();
+  for (int t = 0; t < k; ++t) {
     T bd = ld;
     int bi = li, bl = lr >= 0 ? lane : 64;
-    for (int off = 32; off > 0; off >>= 1) {
-      const T od = __shfl_xor(bd, off);
-      const int oi = __shfl_xor(bi, off);
-      const int ol = __shfl_xor(bl, off);
+    auto combine = [&](T od, int oi, int ol) {
       if (ol < 64 && (bl == 64 || cless(od, oi, bd, bi) || (od == bd && oi == bi && ol < bl))) {
         bd = od;
         bi = oi;
         bl = ol;
       }
+    };
+    combine(dpp_move<0xB1>(bd), dpp_move<0xB1>(bi), dpp_move<0xB1>(bl));     // quad_perm 1,0,3,2
+    combine(dpp_move<0x4E>(bd), dpp_move<0x4E>(bi), dpp_move<0x4E>(bl));     // quad_perm 2,3,0,1
+    combine(dpp_move<0x141>(bd), dpp_move<0x141>(bi), dpp_move<0x141>(bl));  // row_half_mirror
+    combine(dpp_move<0x140>(bd), dpp_move<0x140>(bi), dpp_move<0x140>(bl));  // row_mirror
+    for (int off = 16; off < 64; off <<= 1) {
+      const T od = __shfl_xor(bd, off);
+      const int oi = __shfl_xor(bi, off);
+      const int ol = __shfl_xor(bl, off);
+      combine(od, oi, ol);
     }
     if (lane == bl) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r == lr) live[r] = false;
+      if (nr != -2) {
+        ld = nd;
+        li = ni;
+        lr = nr;
+        nr = -2;
+      } else {
+        rescan();
+      }
     }
     // Rows are written descending, as heap_store does.
     if (lane == 0) {
