@@ -33,6 +33,61 @@ __device__ __forceinline__ float xadd(float a, float b) { return __fadd_rn(a, b)
 __device__ __forceinline__ double xsub(double a, double b) { return __dsub_rn(a, b); }
 __device__ __forceinline__ float xsub(float a, float b) { return __fsub_rn(a, b); }
 
+// The reference oracle's distance between candidate row c and query row q:
+// s = sum_p c_p q_p in p order from 0.0, every product and sum rounded; then
+// (s x (-2) + cn) + qn, no clamp (frame/dgsknn_ref_impl.h:63-72, :91-103).
+__device__ __forceinline__ double oracle_dist(const double *c, const double *q, int d, double cn,
+                                              double qn) {
+  double s = 0.0;
+  int p = 0;
+  for (; p + 4 <= d; p += 4) {
+    const double c0 = c[p], c1 = c[p + 1], c2 = c[p + 2], c3 = c[p + 3];
+    const double q0 = q[p], q1 = q[p + 1], q2 = q[p + 2], q3 = q[p + 3];
+    s = xadd(s, xmul(c0, q0));
+    s = xadd(s, xmul(c1, q1));
+    s = xadd(s, xmul(c2, q2));
+    s = xadd(s, xmul(c3, q3));
+  }
+  for (; p < d; ++p) s = xadd(s, xmul(c[p], q[p]));
+  return xadd(xadd(xmul(s, -2.0), cn), qn);
+}
+
+// FP64 MFMA filter margins.  The MFMA dot product s_m of a pair (a, b) of
+// dimension d, summed over T = 4 KS terms (zero padding included) in any
+// order, and the oracle's sequential sum s_o both lie within
+// gamma_n sum_p |a_p b_p| of the exact dot product (gamma_n = n u / (1 - n u),
+// u = 2^-53, n = T + 8 and d), so
+//   |2 s_m - 2 s_o| <= c (|a|^2 + |b|^2) / 2,  c = 2.5 (T + 8 + d) u
+// (Cauchy-Schwarz, 2 |a||b| <= |a|^2 + |b|^2, 25 % slack for the roundings of
+// the margins themselves and of the squared norms, which are computed from
+// the operands, not taken from the caller).  A kernel then shifts the
+// candidate's norm down by c |a|^2 / 2 (mfma_shift_norm) and the query's
+// threshold up by c |b|^2 / 2 (mfma_threshold): every pair whose oracle
+// distance can reach the row's root passes the filter, and a passing pair's
+// exact distance is formed by oracle_dist.
+__device__ __forceinline__ double mfma_margin_c(int terms, int d) {
+  return 0x1p-53 * 2.5 * (double)(terms + 8 + d);
+}
+
+// fl(cn - e) with e = 1.5 max(c a2 / 2, 4 u |cn|): at least c a2 / 2 below cn
+// after its own rounding, whatever the caller's norm cn.  NaN stays NaN
+// (empty candidate slots carry NaN norms and never pass).
+__device__ __forceinline__ double mfma_shift_norm(double cn, double a2, double c) {
+  const double e = xmul(1.5, fmax(xmul(xmul(0.5, c), a2), xmul(0x1p-51, __builtin_fabs(cn))));
+  return xsub(cn, e);
+}
+
+// Threshold on t = fl(-2 s_m + shifted cn) for a row of root r and query norm
+// qn: a pair whose oracle distance fl(fl(-2 s_o + cn) + qn) is <= r has
+// t <= fl(fl(r - qn) + 2^-48 (|r| + |qn|) + c b2 / 2 + 2^-1000).  The relative
+// term covers the roundings of the distance, of t and of this sum; +inf
+// roots give +inf.
+__device__ __forceinline__ double mfma_threshold(double r, double qn, double half_c_b2) {
+  const double m = xadd(xadd(xmul(0x1p-48, xadd(__builtin_fabs(r), __builtin_fabs(qn))), half_c_b2),
+                        0x1p-1000);
+  return xadd(xsub(r, qn), m);
+}
+
 // Constant-address-space view of a global array: uniform reads through it
 // become scalar loads (SGPR operands shared by the whole wave).  Only arrays
 // a kernel never writes are viewed so.

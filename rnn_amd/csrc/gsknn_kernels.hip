@@ -16,8 +16,10 @@
 //   knn_exact_kernel<T>     one thread per query, VALU, the reference oracle's
 //                           arithmetic in p order (product and sum rounded
 //                           separately): distances bit-identical to it.
-//   knn_mfma_f64_kernel     FP64 MFMA (v_mfma_f64_16x16x4_f64) contraction,
-//                           wave-owned queries, LDS heaps.
+//   knn_mfma_f64_kernel     FP64 MFMA (v_mfma_f64_16x16x4_f64) filter with a
+//                           rigorous error margin, oracle-arithmetic
+//                           distances for the pairs that pass (bit-identical
+//                           results), wave-owned queries, LDS heaps.
 //   knn_merge_kernel<T>     folds the partial lists of a candidate-split
 //                           launch into the resident heap rows.
 //   sqnorm_kernel<T>        X2[i] = sum_p X[i][p]^2 in p order.
@@ -56,11 +58,16 @@ namespace gsknn {
 // fetches them with lane shuffles.  NB = 1: two barriers per tile, norms and
 // ids in LDS.
 //
-// Epilogue per lane: dist = (-2 * dot + XA2[amap[i]]) + XB2[bmap[j]], test
-// against the row's root.  A wave-wide ballot skips the insertion step when
-// no lane has a survivor, which is the common case once the rows are warm.
-// Otherwise the four lane groups (l/16 = 0..3, each covering the 16 queries
-// once) insert in turn, so no two lanes ever touch the same heap at once.
+// Epilogue per lane: t = fl(-2 dot + shifted XA2[amap[i]]) against a per-row
+// threshold with the MFMA error margin folded in (mfma_shift_norm,
+// mfma_threshold), so every pair whose oracle distance can reach the row's
+// root passes.  A wave-wide ballot skips the insertion step when no lane has
+// a survivor, which is the common case once the rows are warm.  Passing
+// pairs wait in a per-lane queue; when one would overflow, every lane forms
+// the oracle-arithmetic distance of its queued pairs (oracle_dist) and the
+// four lane groups (l/16 = 0..3, each covering the 16 queries once) insert
+// in turn, so no two lanes ever touch the same heap at once.  Heaps hold
+// only oracle distances: results are bit-identical to the oracle's.
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -123,6 +130,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 
   if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
 
+  const double cm = mfma_margin_c(4 * KS, d);
   double bq[QB][KS];
   double qn[QB];
   int ql[QB];
@@ -142,6 +150,8 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       bq[qb][s] = (qv[qb] && p < d) ? v : 0.0;
     }
   }
+  // XB row of query block qb.
+  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : a.n - 1]; };
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   // Staging: the tile is BLK fragment blocks of 64 doubles (k-step s,
@@ -204,15 +214,15 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       dst[bidx * 64 + lane] = (ok && p < d) ? stage[e] : 0.0;
     }
     if (nslot) {
-      // An empty slot gets distance +inf and index INT_MAX: it loses to
-      // every row entry, including (+inf, -1) placeholders.
+      // An empty slot gets a NaN norm (no comparison with it passes) and
+      // index INT_MAX.
 #pragma unroll
       for (int r = 0; r < NPW; ++r) {
         const int nb = w + r * W;
         if (nb < CB) {
           const int slot = nb * 16 + si;
           const bool ok = cbeg + t * C + slot < cend;
-          tn[slot] = ok ? stage_n[r] : __builtin_inf();
+          tn[slot] = ok ? stage_n[r] : __builtin_nan("");
           ti[slot] = ok ? own_id(ids, nb) : INT_MAX;
         }
       }
@@ -225,13 +235,13 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     const int c = min(cbeg + t * C + cl, cend - 1);
     return a.amap[c];
   };
-  double cur_tn = __builtin_inf();
+  double cur_tn = __builtin_nan("");
   int cur_ti = INT_MAX;
   int lid_a = 0, lid_b = 0;  // lane ids of tiles t + 1 and t + 2
   if (NB == 2 && ntiles > 0) {
     const int id0 = lane_id(0);
     const bool ok = cbeg + cl < cend;
-    cur_tn = ok ? a.XA2[id0] : __builtin_inf();
+    cur_tn = ok ? a.XA2[id0] : __builtin_nan("");
     cur_ti = ok ? id0 : INT_MAX;
     lid_a = ntiles > 1 ? lane_id(1) : id0;
     lid_b = ntiles > 2 ? lane_id(2) : lid_a;
@@ -250,59 +260,49 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     }
   }
   __syncthreads();
-  // The filter compares t = fl(-2 dot + XA2[amap[i]]) with a per-row
-  // threshold instead of the distance fl(t + qn) with the root r.  If
-  // fl(t + qn) <= r then t + qn <= r + 2u|r| (u = 2^-53), so every such pair
-  // has t <= thr for thr = fl(fl(r - qn) + 2^-50 (|r| + |qn|) + 2^-1020),
-  // whose rounding errors the margin covers.  +inf roots give +inf; lanes
-  // without a query get -inf.  Pairs that pass are tested exactly.
+  // Filter threshold of a query row (mfma_threshold): every pair whose
+  // oracle distance can reach the root r passes; lanes without a query get
+  // -inf.  The query's squared norm comes from its operand registers (the
+  // four lane groups hold disjoint dims of the row); every lane of the wave
+  // calls it.
   auto root_thr = [&](int qb) -> double {
+    double b2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b2 = __builtin_fma(bq[qb][s], bq[qb][s], b2);
+    b2 = xadd(b2, __shfl_xor(b2, 16));
+    b2 = xadd(b2, __shfl_xor(b2, 32));
     if (!qv[qb]) return -__builtin_inf();
-    const double r = hd[ql[qb]];
-    const double m = xadd(xmul(0x1p-50, xadd(__builtin_fabs(r), __builtin_fabs(qn[qb]))), 0x1p-1020);
-    return xadd(xadd(r, -qn[qb]), m);
+    return mfma_threshold(hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
   };
   double thr[QB];
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
 
-  // Per-lane survivor queue: distance and id, with the query block in the
-  // id's top bit (ids are non-negative).
+  // Per-lane queue of passing pairs: candidate id, with the query block in
+  // the top bit (ids are non-negative).
   static_assert(QB <= 2, "query block fits one bit");
-  constexpr int QR = 6;
-  double pq_d[QR];
+  constexpr int QR = 4;
   unsigned pq_i[QR];
   int pq_n = 0;
 #pragma unroll
-  for (int r = 0; r < QR; ++r) {
-    pq_d[r] = 0.0;
-    pq_i[r] = 0u;
-  }
-  // Result value b (= cb * 4 + v) of query block qb: the exact distance
-  // fl(t + |b|^2) from acc's t = fl(-2 dot + |a|^2), and the candidate id
-  // (from the tile's id slots in LDS, or from cid when NB = 2).
-  auto survivor = [&](int qb, int b, const dbl4 (*acc)[CB], const idv cid, double &dist, int &idx)
-                      __attribute__((always_inline)) {
-    double t = acc[qb][0][0];
+  for (int r = 0; r < QR; ++r) pq_i[r] = 0u;
+  // Apply the queues: every lane forms the oracle distances of its queued
+  // pairs at once (rows read from global memory), then the four lane groups,
+  // which cover the wave's queries once each (within a group every lane owns
+  // a distinct heap), update their heaps in turn.
+  auto flush = [&]() __attribute__((always_inline)) {
+    double ed[QR];
+    const int qr0 = qrow(0), qr1 = qrow(QB - 1);
 #pragma unroll
-    for (int bb = 1; bb < CB * 4; ++bb)
-      if (b == bb) t = acc[qb][bb >> 2][bb & 3];
-    dist = xadd(t, qn[qb]);
-    if (NB == 2) {
-      idx = cid[0];
-#pragma unroll
-      for (int bb = 1; bb < CB * 4; ++bb)
-        if (b == bb) idx = cid[bb];
-    } else {
-      idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+    for (int r = 0; r < QR; ++r) {
+      ed[r] = 0.0;
+      if (r < pq_n) {
+        const bool hi_qb = pq_i[r] >> 31;
+        const int id = (int)(pq_i[r] & 0x7fffffffu);
+        const double *xq = a.XB + (size_t)(hi_qb ? qr1 : qr0) * d;
+        ed[r] = oracle_dist(a.XA + (size_t)id * d, xq, d, a.XA2[id], hi_qb ? qn[QB - 1] : qn[0]);
+      }
     }
-  };
-  // Apply the queued survivors and this tile's (pass bits over acc, ids in
-  // the tile's id slots or cid) to the heaps.  The four lane groups cover
-  // the wave's queries once each, so within a group every lane owns a
-  // distinct heap; groups take turns.
-  auto flush = [&](const unsigned *pass, const dbl4 (*acc)[CB], const idv cid)
-                   __attribute__((always_inline)) {
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       if (grp == g) {
@@ -311,20 +311,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
           if (r < pq_n) {
             const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
             const int id = (int)(pq_i[r] & 0x7fffffffu);
-            if (cless(pq_d[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, pq_d[r], id);
-          }
-        }
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass ? pass[qb] : 0u;
-          const int slot = ql[qb];
-          while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            double dist;
-            int idx;
-            survivor(qb, b, acc, cid, dist, idx);
-            if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, ed[r], id);
           }
         }
       }
@@ -337,6 +324,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
   };
 
   dbl4 acc[QB][CB];
+  double a2[CB];  // squared norms of this lane's share of the tile's candidate rows
   // MFMAs of candidate blocks [LO, HI) of the tile at `cur`.
   auto mfma_blocks = [&](const double *cur, auto lo_c, auto hi_c) __attribute__((always_inline)) {
     constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
@@ -345,6 +333,8 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 #pragma unroll
       for (int cb = LO; cb < HI; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
+    for (int cb = LO; cb < HI; ++cb) a2[cb] = 0.0;
+#pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int cb = LO; cb < HI; ++cb) {
@@ -352,20 +342,41 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+        a2[cb] = __builtin_fma(av, av, a2[cb]);
       }
     }
   };
-  // Epilogue of candidate blocks [LO, HI): one FMA and one compare per pair,
-  // the compare's wave mask kept in scalar registers; per-lane bits and
-  // exact distances are built only when some pair passes.  Norms and ids
-  // come from LDS (NB = 1) or from the lanes' registers tnr/tir (NB = 2).
+  // Epilogue of candidate blocks [LO, HI): one FMA and one compare per pair
+  // against the shifted norm and threshold (mfma_shift_norm,
+  // mfma_threshold), the compare's wave mask kept in scalar registers;
+  // per-lane bits and ids are built only when some pair passes.  Norms and
+  // ids come from LDS (NB = 1) or from the lanes' registers tnr/tir (NB = 2:
+  // lane l holds candidate l % C, whose squared norm is in a2 of block
+  // (l / 16) % CB).
   auto epilogue = [&](auto lo_c, auto hi_c, double tnr, int tir) __attribute__((always_inline)) {
     constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
-    double ctn[CB * 4];
+    double s2[CB];
 #pragma unroll
-    for (int b = LO * 4; b < HI * 4; ++b) {
-      const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
-      ctn[b] = NB == 2 ? __shfl(tnr, c) : tn[c];
+    for (int cb = LO; cb < HI; ++cb) {
+      s2[cb] = xadd(a2[cb], __shfl_xor(a2[cb], 16));
+      s2[cb] = xadd(s2[cb], __shfl_xor(s2[cb], 32));
+    }
+    double ctn[CB * 4];
+    if (NB == 2) {
+      double own = s2[LO];
+#pragma unroll
+      for (int cb = LO + 1; cb < HI; ++cb)
+        if (((lane >> 4) & (CB - 1)) == cb) own = s2[cb];
+      const double sn = mfma_shift_norm(tnr, own, cm);
+#pragma unroll
+      for (int b = LO * 4; b < HI * 4; ++b) ctn[b] = __shfl(sn, (b >> 2) * 16 + grp + 4 * (b & 3));
+    } else {
+#pragma unroll
+      for (int cb = LO; cb < HI; ++cb) {
+        const double sn = mfma_shift_norm(tn[cb * 16 + (lane & 15)], s2[cb], cm);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ctn[cb * 4 + v] = __shfl(sn, grp + 4 * v);
+      }
     }
     unsigned long long msk[QB][CB * 4];
     unsigned long long anym = 0;
@@ -406,30 +417,39 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
         for (int b = LO * 4; b < HI * 4; ++b)
           cid[b] = __shfl(tir, (b >> 2) * 16 + grp + 4 * (b & 3));
       }
-      int npass = 0;
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
-      if (__any(pq_n + npass > QR)) {
-        flush(pass, acc, cid);
-      } else {
+      // Queue the passing pairs; whenever some lane's queue is full, apply
+      // the queues and drop the pairs left that fail the new thresholds.
+      while (true) {
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass[qb];
-          while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            double dist;
+          while (pass[qb] && pq_n < QR) {
+            const int b = __builtin_ctz(pass[qb]);
+            pass[qb] &= pass[qb] - 1;
             int idx;
-            survivor(qb, b, acc, cid, dist, idx);
+            if (NB == 2) {
+              idx = cid[0];
+#pragma unroll
+              for (int bb = 1; bb < CB * 4; ++bb)
+                if (b == bb) idx = cid[bb];
+            } else {
+              idx = ti[(b >> 2) * 16 + grp + 4 * (b & 3)];
+            }
 #pragma unroll
             for (int r = 0; r < QR; ++r)
-              if (pq_n == r) {
-                pq_d[r] = dist;
-                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
-              }
+              if (pq_n == r) pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
             ++pq_n;
           }
         }
+        unsigned left = 0u;
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) left |= pass[qb];
+        if (!__any(left != 0u)) break;
+        flush();
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+          for (int b = LO * 4; b < HI * 4; ++b)
+            if (!(acc[qb][b >> 2][b & 3] <= thr[qb])) pass[qb] &= ~(1u << b);
       }
     }
   };
@@ -464,7 +484,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       id_a = id_b;
       id_b = id_c;
       const bool nxt_ok = more && cbeg + (t + 1) * C + cl < cend;
-      cur_tn = nxt_ok ? nxt_tn : __builtin_inf();
+      cur_tn = nxt_ok ? nxt_tn : __builtin_nan("");
       cur_ti = nxt_ok ? lid_a : INT_MAX;
       lid_a = lid_b;
       lid_b = lid_c;
@@ -476,7 +496,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     }
   }
 
-  if (__any(pq_n > 0)) flush(nullptr, acc, idv{});
+  if (__any(pq_n > 0)) flush();
   __syncthreads();
   if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
 }
@@ -506,7 +526,8 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 //    zero query coordinate, so every dim enters the sum exactly once and
 //    every lane reads KS contiguous in-bounds doubles of each candidate row.
 //    The compiler schedules the refills.
-// amap entries are read two tiles ahead, norms one tile ahead.
+// amap entries are read two tiles ahead, norms one tile ahead.  Each operand
+// also feeds the candidate's squared norm (one FMA) before its refill.
 // The result layout, filter, survivor queue and heap updates are those of
 // knn_mfma_f64_kernel; the candidate norms and ids of a lane's result values
 // come from the lanes that loaded them, by lane shuffles.
@@ -537,6 +558,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   const int st = LINE ? 2 * grp : min(own0, d - KS);       // first dim it reads
   // Offset of k-step s from st in either deal.
   auto step_dim = [](int s) -> int { return LINE ? 8 * (s >> 1) + (s & 1) : s; };
+  const double cm = mfma_margin_c(4 * KS, d);
   double bq[QB][KS];
   double qn[QB];
   int ql[QB];
@@ -555,6 +577,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
       bq[qb][s] = (qv[qb] && (LINE || (p >= own0 && p < own1))) ? xq[p] : 0.0;
     }
   }
+  // XB row of query block qb (every lane of the wave calls it).
+  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : a.n - 1]; };
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   struct Ids {
@@ -577,9 +601,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   double cn[CB];  // norms of this lane's candidates in the current tile
   int ci[CB];     // their ids (INT_MAX for empty slots)
   Ids id_a{}, id_b{};
+  // Empty candidate slots carry a NaN norm: no comparison with one passes.
+  const double nan = __builtin_nan("");
 #pragma unroll
   for (int cb = 0; cb < CB; ++cb) {
-    cn[cb] = __builtin_inf();
+    cn[cb] = nan;
     ci[cb] = INT_MAX;
   }
   if (ntiles > 0) {
@@ -590,7 +616,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
       for (int s = 0; s < KS; ++s) xb[s][cb] = load_op(r, s);
       const bool ok = slot_ok(0, cb);
-      cn[cb] = ok ? a.XA2[id0.v[cb]] : __builtin_inf();
+      cn[cb] = ok ? a.XA2[id0.v[cb]] : nan;
       ci[cb] = ok ? id0.v[cb] : INT_MAX;
     }
     id_a = ntiles > 1 ? cand_id(1) : id0;
@@ -598,42 +624,48 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   }
   __syncthreads();  // heaps loaded
 
-  // Filter threshold of a query row, as in knn_mfma_f64_kernel: a pair with
-  // fl(t + qn) <= r has t <= fl(fl(r - qn) + 2^-50 (|r| + |qn|) + 2^-1020).
+  // Filter threshold of a query row (mfma_threshold): every pair whose
+  // oracle distance can reach the root r passes; lanes without a query get
+  // -inf.  The query's squared norm comes from its operand registers (the
+  // four lane groups hold disjoint dims of the row); every lane of the wave
+  // calls it.
   auto root_thr = [&](int qb) -> double {
+    double b2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b2 = __builtin_fma(bq[qb][s], bq[qb][s], b2);
+    b2 = xadd(b2, __shfl_xor(b2, 16));
+    b2 = xadd(b2, __shfl_xor(b2, 32));
     if (!qv[qb]) return -__builtin_inf();
-    const double r = hd[ql[qb]];
-    const double m = xadd(xmul(0x1p-50, xadd(__builtin_fabs(r), __builtin_fabs(qn[qb]))), 0x1p-1020);
-    return xadd(xadd(r, -qn[qb]), m);
+    return mfma_threshold(hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
   };
   double thr[QB];
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) thr[qb] = root_thr(qb);
 
-  constexpr int QR = 6;
-  double pq_d[QR];
+  // Per-lane queue of passing pairs: candidate id, query block in the top
+  // bit (ids are non-negative).
+  constexpr int QR = 4;
   unsigned pq_i[QR];
   int pq_n = 0;
 #pragma unroll
-  for (int r = 0; r < QR; ++r) {
-    pq_d[r] = 0.0;
-    pq_i[r] = 0u;
-  }
+  for (int r = 0; r < QR; ++r) pq_i[r] = 0u;
   dbl4 acc[QB][CB];
-  // Result value b = cb * 4 + v of a lane is the pair (query row ql[qb],
-  // candidate cb * 16 + grp + 4 v); its exact distance fl(t + qn).
-  auto survivor = [&](int qb, int b, const idv cid, double &dist, int &idx) __attribute__((always_inline)) {
-    double t = acc[qb][0][0];
-    idx = cid[0];
+  // Apply the queues: every lane forms the oracle distances of its queued
+  // pairs at once (rows read from global memory), then the four lane groups,
+  // which cover the wave's queries once each, update their heaps in turn.
+  auto flush = [&]() __attribute__((always_inline)) {
+    double ed[QR];
+    const int qr0 = qrow(0), qr1 = qrow(QB - 1);
 #pragma unroll
-    for (int bb = 1; bb < CB * 4; ++bb)
-      if (b == bb) {
-        t = acc[qb][bb >> 2][bb & 3];
-        idx = cid[bb];
+    for (int r = 0; r < QR; ++r) {
+      ed[r] = 0.0;
+      if (r < pq_n) {
+        const bool hi_qb = pq_i[r] >> 31;
+        const int id = (int)(pq_i[r] & 0x7fffffffu);
+        const double *xq = a.XB + (size_t)(hi_qb ? qr1 : qr0) * d;
+        ed[r] = oracle_dist(a.XA + (size_t)id * d, xq, d, a.XA2[id], hi_qb ? qn[QB - 1] : qn[0]);
       }
-    dist = xadd(t, qn[qb]);
-  };
-  auto flush = [&](const unsigned *pass, const idv cid) __attribute__((always_inline)) {
+    }
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       if (grp == g) {
@@ -642,20 +674,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
           if (r < pq_n) {
             const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
             const int id = (int)(pq_i[r] & 0x7fffffffu);
-            if (cless(pq_d[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, pq_d[r], id);
-          }
-        }
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass ? pass[qb] : 0u;
-          const int slot = ql[qb];
-          while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            double dist;
-            int idx;
-            survivor(qb, b, cid, dist, idx);
-            if (cless(dist, idx, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, dist, idx);
+            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, ed[r], id);
           }
         }
       }
@@ -680,11 +699,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double a2[CB];  // squared norms of this lane's share of the candidate rows
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) a2[cb] = 0.0;
     // Each operand register is refilled with tile t + 1's value once the
-    // MFMAs of tile t have read it (on the last tile the refill reads valid
-    // rows and is never used).  The wave issues its MFMAs at raised priority,
-    // so of the two waves of a SIMD the one in its MFMA phase keeps the
-    // matrix pipe while the other runs its epilogue.
+    // MFMAs of tile t (and the squared-norm FMA) have read it (on the last
+    // tile the refill reads valid rows and is never used).  The wave issues
+    // its MFMAs at raised priority, so of the two waves of a SIMD the one in
+    // its MFMA phase keeps the matrix pipe while the other runs its epilogue.
     __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -694,6 +716,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+        a2[cb] = __builtin_fma(av, av, a2[cb]);
         if (!LINE) xb[s][cb] = load_op(nr[cb], s);
       }
       if (LINE) {
@@ -710,12 +733,20 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
     __builtin_amdgcn_s_setprio(0);
 
-    // Epilogue: one FMA and one compare per pair; wave masks in scalar
-    // registers; per-lane bits, exact distances and ids only when a pair
-    // passes.
+    // Epilogue: one FMA and one compare per pair against the shifted norm
+    // and threshold (mfma_shift_norm, mfma_threshold); wave masks in scalar
+    // registers; per-lane bits and ids only when a pair passes.  The lane
+    // groups hold disjoint (LINE) or overlapping (contiguous runs: an upper
+    // bound) dims of each candidate row.
     double ctn[CB * 4];
 #pragma unroll
-    for (int b = 0; b < CB * 4; ++b) ctn[b] = __shfl(cn[b >> 2], grp + 4 * (b & 3));
+    for (int cb = 0; cb < CB; ++cb) {
+      double s2 = xadd(a2[cb], __shfl_xor(a2[cb], 16));
+      s2 = xadd(s2, __shfl_xor(s2, 32));
+      const double sn = mfma_shift_norm(cn[cb], s2, cm);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ctn[cb * 4 + v] = __shfl(sn, grp + 4 * v);
+    }
     unsigned long long msk[QB][CB * 4];
     unsigned long long anym = 0;
 #pragma unroll
@@ -745,44 +776,48 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
       idv cid;
 #pragma unroll
       for (int b = 0; b < CB * 4; ++b) cid[b] = __shfl(ci[b >> 2], grp + 4 * (b & 3));
-      int npass = 0;
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) npass += __builtin_popcount(pass[qb]);
-      if (__any(pq_n + npass > QR)) {
-        flush(pass, cid);
-      } else {
+      // Queue the passing pairs; whenever some lane's queue is full, apply
+      // the queues and drop the pairs left that fail the new thresholds.
+      while (true) {
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass[qb];
-          while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            double dist;
-            int idx;
-            survivor(qb, b, cid, dist, idx);
+          while (pass[qb] && pq_n < QR) {
+            const int b = __builtin_ctz(pass[qb]);
+            pass[qb] &= pass[qb] - 1;
+            int idx = cid[0];
+#pragma unroll
+            for (int bb = 1; bb < CB * 4; ++bb)
+              if (b == bb) idx = cid[bb];
 #pragma unroll
             for (int r = 0; r < QR; ++r)
-              if (pq_n == r) {
-                pq_d[r] = dist;
-                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
-              }
+              if (pq_n == r) pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
             ++pq_n;
           }
         }
+        unsigned left = 0u;
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) left |= pass[qb];
+        if (!__any(left != 0u)) break;
+        flush();
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+          for (int b = 0; b < CB * 4; ++b)
+            if (!(acc[qb][b >> 2][b & 3] <= thr[qb])) pass[qb] &= ~(1u << b);
       }
     }
 
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
       const bool ok = t + 1 < ntiles && slot_ok(t + 1, cb);
-      cn[cb] = ok ? nn[cb] : __builtin_inf();
+      cn[cb] = ok ? nn[cb] : nan;
       ci[cb] = ok ? id_a.v[cb] : INT_MAX;
     }
     id_a = id_b;
     id_b = id_c;
   }
 
-  if (__any(pq_n > 0)) flush(nullptr, idv{});
+  if (__any(pq_n > 0)) flush();
   __syncthreads();
   if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
 }

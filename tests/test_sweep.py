@@ -96,6 +96,45 @@ def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_sweep_config4_parameters(tmp_path, monkeypatch):
+    """The config-4 sweep parameters (k = 32, leaves of 256..4096 points,
+    d = 32, 256-cluster mixture) at N = 2^16: exact median splits of a power
+    of two give leaves of 256, 512, 1024 and 2048 points (a 4096-point node
+    always lies above its stop threshold, drawn below 4096).  Two iterations
+    (a direct launch, then a graph replay), each equal bitwise to the
+    composition of per-leaf oracle calls."""
+    import torch
+    monkeypatch.setenv("GSKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    dev = torch.device("cuda")
+    N, d, k = 1 << 16, 32, 32
+    Xn = ds.config4_points(N, d, 256, seed=3)
+    X = torch.from_numpy(Xn).to(dev)
+    sw = sweep.RPTreeSweep(X, k, seed=3, leaf_lo=256, leaf_hi=4096)
+    X2 = oracle.sqnorm(Xn)
+    Dr = np.full((N, k), np.inf, dtype=np.float32)
+    Ir = np.full((N, k), -1, dtype=np.int32)
+    biggest = 0
+    for _ in range(2):
+        perm, leaves, info = sw.step()
+        print(f"[gpu] config-4 sweep iteration {info['iteration']}: {len(leaves)} leaves", flush=True)
+        sizes = [n for _, n in leaves]
+        assert min(sizes) >= 128 and max(sizes) <= 4096
+        biggest = max(biggest, max(sizes))
+        perm = perm.cpu().numpy()
+        for s, n in leaves:
+            ids = np.ascontiguousarray(perm[s:s + n])
+            Ds, Is = np.ascontiguousarray(Dr[ids]), np.ascontiguousarray(Ir[ids])
+            oracle.gsknn(Xn, X2, ids, Xn, X2, ids, Ds, Is, canonical=True)
+            Dr[ids], Ir[ids] = Ds, Is
+        a = oracle.sort_rows(sw.D.cpu().numpy(), sw.I.cpu().numpy())
+        b = oracle.sort_rows(Dr, Ir)
+        np.testing.assert_array_equal(a[1], b[1])
+        assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
+    assert biggest == 2048
+    assert sw.graph is not None
+
+
+@pytest.mark.gpu
 def test_device_tree_partition_and_median():
     # The device build (fused per-level key kernel) partitions the points
     # into bounded leaves, and its root split is the median of the
