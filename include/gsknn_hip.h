@@ -63,6 +63,24 @@ int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA,
                    int ldk, const int *hrow, int mode, void *work,
                    size_t work_bytes, void *stream);
 
+/* gsknn_dknn_noalloc_dev: gsknn_dknn_dev without the work arguments, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The call allocates nothing: the workspace a launch needs (partial lists
+ * when the candidates are split over workgroups, i.e.
+ * gsknn_dknn_work_bytes(m, n, d, k, mode) bytes) comes from a fixed device
+ * array of GSKNN_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; a call needing more fails with GSKNN_ELIMIT.  Results equal
+ * gsknn_dknn_dev's.  All calls on one GPU share its array: calls that
+ * overlap on one GPU (other streams or threads) race and must be
+ * serialised by the caller. */
+#define GSKNN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int gsknn_dknn_noalloc_dev(int m, int n, int d, int k, const double *XA,
+                           const double *XA2, const int *amap,
+                           const double *XB, const double *XB2,
+                           const int *bmap, double *D, int *I, int ldk,
+                           const int *hrow, int mode, void *stream);
+
 size_t gsknn_sknn_work_bytes(int m, int n, int d, int k, int mode);
 int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
                    const float *XA2, const int *amap, const float *XB,

@@ -21,7 +21,7 @@ LIB = os.path.join(LIBDIR, "libgsknn.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip",
-           "gsknn_l1.hip", "gsknn_api.cpp", "gsknn_serve.cpp"]
+           "gsknn_l1.hip", "gsknn_noalloc.hip", "gsknn_api.cpp"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",
@@ -35,7 +35,7 @@ HIPFLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    return [os.path.join(CSRC, s) for s in SOURCES]
 
 
 def build(verbose: bool = False) -> str:

@@ -63,11 +63,13 @@ static int check_sizes(const char *fn, int m, int n, int d, int k, int ldk) {
   return GSKNN_OK;
 }
 
+// allow_alloc = false: the caller's workspace must suffice (no allocation of
+// any kind happens in the call).
 template <typename T>
 static int knn_dev(const char *fn, int m, int n, int d, int k, const T *XA, const T *XA2,
                    const int *amap, const T *XB, const T *XB2, const int *bmap, T *D, int *I,
                    int ldk, const int *hrow, int mode, void *work, size_t work_bytes,
-                   void *stream) {
+                   void *stream, bool allow_alloc = true) {
   int rc = check_sizes(fn, m, n, d, k, ldk);
   if (rc) return rc;
   if (m == 0 || n == 0 || d == 0 || k == 0) return GSKNN_OK;  // gsknn.c:909-913
@@ -76,6 +78,9 @@ static int knn_dev(const char *fn, int m, int n, int d, int k, const T *XA, cons
   hipStream_t s = (hipStream_t)stream;
   const size_t need = knn_work_bytes(m, n, d, k, (int)sizeof(T), mode);
   void *own = nullptr;
+  if (!allow_alloc && need > work_bytes)
+    return set_err(GSKNN_ELIMIT, "%s: the call needs %zu bytes of scratch, more than the %zu available",
+                   fn, need, work_bytes);
   if (need > 0 && (work == nullptr || work_bytes < need)) {
     if (work != nullptr)
       return set_err(GSKNN_EINVAL, "%s: workspace %zu < %zu bytes", fn, work_bytes, need);
@@ -125,6 +130,21 @@ extern "C" int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA, const
   if (mode < 0 || mode > 3) return set_err(GSKNN_EINVAL, "gsknn_sknn_dev: mode %d", mode);
   return knn_dev("gsknn_sknn_dev", m, n, d, k, XA, XA2, amap, XB, XB2, bmap, D, I, ldk, hrow,
                  mode, work, work_bytes, stream);
+}
+
+int gsknn::dknn_dev_fixed_work(const char *fn, int m, int n, int d, int k, const double *XA,
+                               const double *XA2, const int *amap, const double *XB,
+                               const double *XB2, const int *bmap, double *D, int *I, int ldk,
+                               const int *hrow, int mode, void *work, size_t work_bytes,
+                               void *stream) {
+  g_err[0] = 0;
+  if (mode < 0 || mode > 3) return set_err(GSKNN_EINVAL, "%s: mode %d", fn, mode);
+  return knn_dev(fn, m, n, d, k, XA, XA2, amap, XB, XB2, bmap, D, I, ldk, hrow, mode, work,
+                 work_bytes, stream, false);
+}
+
+int gsknn::api_error(int code, const char *fn, const char *what, int value) {
+  return set_err(code, "%s: %s (%d)", fn, what, value);
 }
 
 template <typename T>

@@ -109,6 +109,15 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
                            const int *bmap, double *D, int *I, int ldk, const int *hrow,
                            void *work, size_t work_bytes, int *overflow_out, hipStream_t s);
 
+// gsknn_dknn_dev with a fixed workspace and no allocation of any kind: a
+// call needing more than work_bytes fails with GSKNN_ELIMIT (gsknn_api.cpp).
+int dknn_dev_fixed_work(const char *fn, int m, int n, int d, int k, const double *XA,
+                        const double *XA2, const int *amap, const double *XB, const double *XB2,
+                        const int *bmap, double *D, int *I, int ldk, const int *hrow, int mode,
+                        void *work, size_t work_bytes, void *stream);
+// Sets gsknn_last_error() to "fn: what (value)" and returns code.
+int api_error(int code, const char *fn, const char *what, int value);
+
 hipError_t launch_sqnorm_f64(int N, int d, const double *X, double *X2, hipStream_t s);
 hipError_t launch_sqnorm_f32(int N, int d, const float *X, float *X2, hipStream_t s);
 

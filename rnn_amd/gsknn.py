@@ -69,6 +69,8 @@ def _declare(L):
     L.gsknn_last_error.restype = ctypes.c_char_p
     L.gsknn_dknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
     L.gsknn_dknn_dev.restype = i32
+    L.gsknn_dknn_noalloc_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp]
+    L.gsknn_dknn_noalloc_dev.restype = i32
     L.gsknn_sknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
     L.gsknn_sknn_dev.restype = i32
     L.gsknn_dknn_work_bytes.argtypes = [i32] * 5
@@ -255,6 +257,20 @@ def dknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work
                               _dp(work), wb, _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20  # GSKNN_NOALLOC_SCRATCH_BYTES (include/gsknn_hip.h)
+
+
+def dknn_noalloc_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, stream=None):
+    """gsknn_dknn_noalloc_dev: dknn_dev without any allocation (workspace
+    from the library's fixed per-GPU scratch array); returns the C return
+    code (0 on success) instead of raising."""
+    k = D.shape[1]
+    m, n, d = amap.numel(), bmap.numel(), XA.shape[1]
+    return lib().gsknn_dknn_noalloc_dev(m, n, d, k, _dp(XA), _dp(XA2), _dp(amap), _dp(XB),
+                                        _dp(XB2), _dp(bmap), _dp(D), _dp(I), D.stride(0),
+                                        _dp(hrow), mode, _stream_ptr(stream))
 
 
 def sknn_dev(XA, XA2, amap, XB, XB2, bmap, D, I, hrow=None, mode=MODE_AUTO, work=None,
