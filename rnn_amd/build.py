@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgsknn.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["gsknn_kernels.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip",
+SOURCES = ["gsknn_kernels.hip", "gsknn_ring.hip", "gsknn_rerank.hip", "gsknn_f32.hip", "gsknn_sweep.hip",
            "gsknn_l1.hip", "gsknn_noalloc.hip", "gsknn_api.cpp"]
 
 HIPFLAGS = [

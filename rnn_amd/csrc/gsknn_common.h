@@ -28,6 +28,9 @@ struct KnnArgs {
   T *partD;         // ascending partial lists (nsplit > 1): [nsplit][k][n], or
   int *partI;       // [n][nsplit][k] when part_qmajor
   int part_qmajor;  // partial lists query-major (read by the wave merge)
+  const T *SN;      // FP64 MFMA: candidate position i's norm XA2[amap[i]]
+                    // shifted down by the MFMA error margin (mfma_shift_norm,
+                    // from the row's squared norm computed by the library)
 };
 
 enum Mode {
@@ -44,6 +47,18 @@ hipError_t launch_knn_f32(const KnnArgs<float> &a, int mode, hipStream_t s,
                           void *work, size_t work_bytes);
 // Workspace bytes the launch above needs (partial lists of split launches).
 size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
+
+// FP64 MFMA operand-ring kernel (gsknn_ring.hip), d <= 256: queries per
+// workgroup, candidates per tile, LDS bytes, workgroups per CU that its
+// registers allow (0 when the instantiation spills; vec = 16-byte row
+// loads), and the launch (a.nsplit/a.cps set, grid gx x nsplit).
+bool ring_supported(int d);
+int ring_queries_per_block(int d);
+int ring_candidates_per_tile(int d);
+size_t ring_lds_bytes(int d, int k);
+int ring_reg_blocks(int d, bool vec);
+int ring_terms(int d);  // MFMA terms per dot product (8 x pairs)
+hipError_t launch_knn_ring(const KnnArgs<double> &a, bool vec, int gx, size_t lds, hipStream_t s);
 
 // Merge of split partial lists into the heap rows (gsknn_kernels.hip);
 // supported when a workgroup's heaps fit in LDS.

@@ -130,7 +130,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 
   if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
 
-  const double cm = mfma_margin_c(4 * KS, d);
+  const double cm = mfma_margin_c(4 * KS, d);  // margin constant (query side)
   double bq[QB][KS];
   double qn[QB];
   int ql[QB];
@@ -189,7 +189,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       if (nb == cb) r = ids.v[cb];
     return r;
   };
-  auto load_x = [&](const Ids &ids) {
+  auto load_x = [&](const Ids &ids, int t) {
 #pragma unroll
     for (int e = 0; e < BPW; ++e) {
       const int bidx = w * BPW + e;
@@ -200,7 +200,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
     if (nslot) {
 #pragma unroll
       for (int r = 0; r < NPW; ++r)
-        if (w + r * W < CB) stage_n[r] = a.XA2[own_id(ids, w + r * W)];
+        if (w + r * W < CB) stage_n[r] = a.SN[min(cbeg + t * C + (w + r * W) * 16 + si, cend - 1)];
     }
   };
   auto store_x = [&](int buf, int t, const Ids &ids) {
@@ -241,7 +241,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
   if (NB == 2 && ntiles > 0) {
     const int id0 = lane_id(0);
     const bool ok = cbeg + cl < cend;
-    cur_tn = ok ? a.XA2[id0] : __builtin_nan("");
+    cur_tn = ok ? a.SN[cbeg + cl] : __builtin_nan("");
     cur_ti = ok ? id0 : INT_MAX;
     lid_a = ntiles > 1 ? lane_id(1) : id0;
     lid_b = ntiles > 2 ? lane_id(2) : lid_a;
@@ -250,12 +250,12 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
   Ids id_a{}, id_b{};  // staging ids of the tiles after the current one
   if (ntiles > 0) {
     const Ids id0 = cand_id(0);
-    load_x(id0);
+    load_x(id0, 0);
     id_a = ntiles > 1 ? cand_id(1) : id0;
     __syncthreads();  // heaps loaded
     store_x(0, 0, id0);
     if (NB == 2) {
-      if (ntiles > 1) load_x(id_a);
+      if (ntiles > 1) load_x(id_a, 1);
       id_b = ntiles > 2 ? cand_id(2) : id_a;
     }
   }
@@ -324,7 +324,6 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
   };
 
   dbl4 acc[QB][CB];
-  double a2[CB];  // squared norms of this lane's share of the tile's candidate rows
   // MFMAs of candidate blocks [LO, HI) of the tile at `cur`.
   auto mfma_blocks = [&](const double *cur, auto lo_c, auto hi_c) __attribute__((always_inline)) {
     constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
@@ -333,8 +332,6 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 #pragma unroll
       for (int cb = LO; cb < HI; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int cb = LO; cb < HI; ++cb) a2[cb] = 0.0;
-#pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int cb = LO; cb < HI; ++cb) {
@@ -342,41 +339,21 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
-        a2[cb] = __builtin_fma(av, av, a2[cb]);
       }
     }
   };
   // Epilogue of candidate blocks [LO, HI): one FMA and one compare per pair
-  // against the shifted norm and threshold (mfma_shift_norm,
-  // mfma_threshold), the compare's wave mask kept in scalar registers;
-  // per-lane bits and ids are built only when some pair passes.  Norms and
-  // ids come from LDS (NB = 1) or from the lanes' registers tnr/tir (NB = 2:
-  // lane l holds candidate l % C, whose squared norm is in a2 of block
-  // (l / 16) % CB).
+  // against the shifted norm (a.SN) and threshold (mfma_threshold), the
+  // compare's wave mask kept in scalar registers; per-lane bits and ids are
+  // built only when some pair passes.  Norms and ids come from LDS (NB = 1)
+  // or from the lanes' registers tnr/tir (NB = 2).
   auto epilogue = [&](auto lo_c, auto hi_c, double tnr, int tir) __attribute__((always_inline)) {
     constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
-    double s2[CB];
-#pragma unroll
-    for (int cb = LO; cb < HI; ++cb) {
-      s2[cb] = xadd(a2[cb], __shfl_xor(a2[cb], 16));
-      s2[cb] = xadd(s2[cb], __shfl_xor(s2[cb], 32));
-    }
     double ctn[CB * 4];
-    if (NB == 2) {
-      double own = s2[LO];
 #pragma unroll
-      for (int cb = LO + 1; cb < HI; ++cb)
-        if (((lane >> 4) & (CB - 1)) == cb) own = s2[cb];
-      const double sn = mfma_shift_norm(tnr, own, cm);
-#pragma unroll
-      for (int b = LO * 4; b < HI * 4; ++b) ctn[b] = __shfl(sn, (b >> 2) * 16 + grp + 4 * (b & 3));
-    } else {
-#pragma unroll
-      for (int cb = LO; cb < HI; ++cb) {
-        const double sn = mfma_shift_norm(tn[cb * 16 + (lane & 15)], s2[cb], cm);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) ctn[cb * 4 + v] = __shfl(sn, grp + 4 * v);
-      }
+    for (int b = LO * 4; b < HI * 4; ++b) {
+      const int c = (b >> 2) * 16 + grp + 4 * (b & 3);
+      ctn[b] = NB == 2 ? __shfl(tnr, c) : tn[c];
     }
     unsigned long long msk[QB][CB * 4];
     unsigned long long anym = 0;
@@ -468,12 +445,12 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
       // (tile t - 1) passed the barrier at the end of the previous step.
       if (more) store_x((t + 1) & 1, t + 1, id_a);
       id_c = (t + 3 < ntiles) ? cand_id(t + 3) : id_b;
-      if (t + 2 < ntiles) load_x(id_b);
+      if (t + 2 < ntiles) load_x(id_b, t + 2);
       lid_c = (t + 3 < ntiles) ? lane_id(t + 3) : lid_b;
-      nxt_tn = a.XA2[lid_a];
+      nxt_tn = a.SN[min(cbeg + (t + 1) * C + cl, cend - 1)];
     } else {
       id_c = (t + 2 < ntiles) ? cand_id(t + 2) : id_a;
-      if (more) load_x(id_a);
+      if (more) load_x(id_a, t + 1);
     }
 
     mfma_blocks(cur, I0{}, ICB{});
@@ -526,8 +503,8 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 //    zero query coordinate, so every dim enters the sum exactly once and
 //    every lane reads KS contiguous in-bounds doubles of each candidate row.
 //    The compiler schedules the refills.
-// amap entries are read two tiles ahead, norms one tile ahead.  Each operand
-// also feeds the candidate's squared norm (one FMA) before its refill.
+// amap entries are read two tiles ahead, shifted norms (a.SN, by position)
+// one tile ahead.
 // The result layout, filter, survivor queue and heap updates are those of
 // knn_mfma_f64_kernel; the candidate norms and ids of a lane's result values
 // come from the lanes that loaded them, by lane shuffles.
@@ -598,9 +575,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   };
   auto load_op = [&](const double *r, int s) -> double { return r[step_dim(s)]; };
 
-  double cn[CB];  // norms of this lane's candidates in the current tile
+  double cn[CB];  // shifted norms (a.SN) of this lane's candidates in the current tile
   int ci[CB];     // their ids (INT_MAX for empty slots)
   Ids id_a{}, id_b{};
+  auto pos_sn = [&](int t, int cb) -> double { return a.SN[min(cbeg + t * C + cb * 16 + cl, cend - 1)]; };
   // Empty candidate slots carry a NaN norm: no comparison with one passes.
   const double nan = __builtin_nan("");
 #pragma unroll
@@ -616,7 +594,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
       for (int s = 0; s < KS; ++s) xb[s][cb] = load_op(r, s);
       const bool ok = slot_ok(0, cb);
-      cn[cb] = ok ? a.XA2[id0.v[cb]] : nan;
+      cn[cb] = ok ? pos_sn(0, cb) : nan;
       ci[cb] = ok ? id0.v[cb] : INT_MAX;
     }
     id_a = ntiles > 1 ? cand_id(1) : id0;
@@ -688,9 +666,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 
   for (int t = 0; t < ntiles; ++t) {
     const Ids id_c = (t + 3 < ntiles) ? cand_id(t + 3) : id_b;
-    double nn[CB];  // norms of tile t + 1 (its ids are id_a)
+    double nn[CB];  // shifted norms of tile t + 1 (its ids are id_a)
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) nn[cb] = a.XA2[id_a.v[cb]];
+    for (int cb = 0; cb < CB; ++cb) nn[cb] = pos_sn(t + 1 < ntiles ? t + 1 : t, cb);
     const double *nr[CB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) nr[cb] = row(id_a, cb);
@@ -699,11 +677,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double a2[CB];  // squared norms of this lane's share of the candidate rows
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb) a2[cb] = 0.0;
     // Each operand register is refilled with tile t + 1's value once the
-    // MFMAs of tile t (and the squared-norm FMA) have read it (on the last
+    // MFMAs of tile t have read it (on the last
     // tile the refill reads valid rows and is never used).  The wave issues
     // its MFMAs at raised priority, so of the two waves of a SIMD the one in
     // its MFMA phase keeps the matrix pipe while the other runs its epilogue.
@@ -716,7 +691,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb)
           acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
-        a2[cb] = __builtin_fma(av, av, a2[cb]);
         if (!LINE) xb[s][cb] = load_op(nr[cb], s);
       }
       if (LINE) {
@@ -734,19 +708,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     __builtin_amdgcn_s_setprio(0);
 
     // Epilogue: one FMA and one compare per pair against the shifted norm
-    // and threshold (mfma_shift_norm, mfma_threshold); wave masks in scalar
-    // registers; per-lane bits and ids only when a pair passes.  The lane
-    // groups hold disjoint (LINE) or overlapping (contiguous runs: an upper
-    // bound) dims of each candidate row.
+    // (a.SN) and threshold (mfma_threshold); wave masks in scalar registers;
+    // per-lane bits and ids only when a pair passes.
     double ctn[CB * 4];
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      double s2 = xadd(a2[cb], __shfl_xor(a2[cb], 16));
-      s2 = xadd(s2, __shfl_xor(s2, 32));
-      const double sn = mfma_shift_norm(cn[cb], s2, cm);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) ctn[cb * 4 + v] = __shfl(sn, grp + 4 * v);
-    }
+    for (int b = 0; b < CB * 4; ++b) ctn[b] = __shfl(cn[b >> 2], grp + 4 * (b & 3));
     unsigned long long msk[QB][CB * 4];
     unsigned long long anym = 0;
 #pragma unroll
@@ -1252,6 +1218,27 @@ __global__ void sqnorm_kernel(int N, int d, const T *X, T *X2) {
   X2[i] = s;
 }
 
+// Shifted candidate norms by position for the FP64 MFMA filter:
+// SN[i] = mfma_shift_norm(XA2[amap[i]], |XA[amap[i]]|^2, c) with c the
+// margin constant of `terms` MFMA terms.  One wave per row (coalesced row
+// loads, a wave sum; any summation order, the margin has slack for it).
+__global__ __launch_bounds__(256) void cand_shift_kernel(int m, int d, int terms, const double *XA,
+                                                         const double *XA2, const int *amap,
+                                                         double *SN) {
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * 4;
+  const double cm = mfma_margin_c(terms, d);
+  for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < m; i += nw) {
+    const int id = amap[i];
+    const double *x = XA + (size_t)id * d;
+    double s = 0.0;
+    for (int p = lane; p < d; p += 64) s = __builtin_fma(x[p], x[p], s);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) SN[i] = mfma_shift_norm(XA2[id], s, cm);
+  }
+}
+
 // dgssq2nrm: one thread per (candidate i, query j), the oracle's arithmetic.
 __global__ void sq2nrm_kernel(int m, int n, int d, const double *XA, const double *XA2,
                               const int *amap, const double *XB, const double *XB2,
@@ -1314,7 +1301,8 @@ struct Plan {
   int qpb = 64;  // queries per workgroup
   int ct = 0;    // exact: candidates per LDS tile (0: scalar-candidate form)
   int ks = 0, qb = 0, w = 0, nb = 1;  // mfma
-  bool direct = false;                // mfma: the direct form (no LDS tile)
+  bool direct = false;                // mfma: a per-wave form (no LDS tile)
+  bool ring = false;                  // mfma: the operand-ring form (gsknn_ring.hip)
   int per_cu = 0;                     // direct form: resident workgroups per CU
   int small = 0;                      // exact, one wave per query: candidate slots per lane
   size_t lds = 0;
@@ -1498,7 +1486,34 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       return e ? atoi(e) : 1;
     }();
     const bool direct_on = direct_env != 0;
-    if (direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 32 && d >= ks) {
+    // The operand-ring form for 64 < d <= 256 (measured faster than the
+    // direct and LDS-tile forms there; slower at d <= 64, where the direct
+    // form keeps two waves per SIMD) wherever its heaps fit and it does not
+    // spill.  GSKNN_MFMA_RING=0 turns it off, =2 takes it for every d it
+    // supports (tuning).
+    static const int ring_env = [] {
+      const char *e = getenv("GSKNN_MFMA_RING");
+      return e ? atoi(e) : 1;
+    }();
+    if (direct_on && ring_env != 0 && (d > 64 || ring_env == 2) && pinned < 0 &&
+        mode != MODE_MFMA_TILE && ring_supported(d)) {
+      const size_t b = ring_lds_bytes(d, k);
+      if (b <= kLdsMax) {
+        const int blocks = std::min((int)(kLdsMax / b), ring_reg_blocks(d, d % 2 == 0));
+        if (blocks > 0) {
+          p.direct = true;
+          p.ring = true;
+          p.qb = ring_queries_per_block(d) / 16;
+          p.w = 1;
+          p.nb = 0;
+          p.qpb = 16 * p.qb;
+          p.lds = b;
+          p.per_cu = blocks;
+          tile_c = ring_candidates_per_tile(d);
+        }
+      }
+    }
+    if (!p.ring && direct_on && pinned < 0 && mode != MODE_MFMA_TILE && ks <= 32 && d >= ks) {
       const int qb = direct_qb(ks), Q = 16 * qb;
       const size_t b = mfma_direct_lds(Q, k);
       if (b <= kLdsMax) {
@@ -1518,8 +1533,8 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     }
     if (p.qb == 0) p.kind = 0;  // heap too large for LDS: exact kernel
     if (verbose())
-      fprintf(stderr, "gsknn: mfma plan d=%d k=%d: ks=%d qb=%d w=%d nb=%d direct=%d lds=%zu score=%d\n",
-              d, k, ks, p.qb, p.w, p.nb, (int)p.direct, p.lds, best);
+      fprintf(stderr, "gsknn: mfma plan d=%d k=%d: ks=%d qb=%d w=%d nb=%d direct=%d ring=%d lds=%zu score=%d\n",
+              d, k, ks, p.qb, p.w, p.nb, (int)p.direct, (int)p.ring, p.lds, best);
   }
   if (p.kind == 0) {
     // d in {8, 16, 32, 64}: the scalar-candidate form (heaps only in LDS).
@@ -1747,6 +1762,10 @@ size_t part_bytes(const Plan &p, int n, int k, int elem) {
   return al16(ents * elem) + al16(ents * sizeof(int));
 }
 
+// Workspace of an FP64 MFMA launch: the shifted candidate norms (m doubles),
+// then the split partial lists.
+size_t a2_bytes(const Plan &p, int m) { return p.kind == 1 ? al16(sizeof(double) * (size_t)m) : 0; }
+
 // FP32 calls with d >= 8 and at least 64 candidates take the FP32 MFMA
 // candidate pass + oracle-arithmetic re-rank; the rest the exact kernel.
 bool use_f32_mfma(int m, int d, int k, int mode) {
@@ -1764,15 +1783,34 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
   a.nsplit = p.nsplit;
   a.cps = p.cps;
   a.part_qmajor = p.nsplit > 1 && merge_query_major(a.n, a.k, p.nsplit);
+  const size_t a2b = a2_bytes(p, a.m);
+  const size_t need = a2b + part_bytes(p, a.n, a.k, (int)sizeof(T));
+  if (need > 0 && (work == nullptr || work_bytes < need)) return hipErrorInvalidValue;
   if (p.nsplit > 1) {
-    const size_t need = part_bytes(p, a.n, a.k, (int)sizeof(T));
-    if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
     const size_t ents = (size_t)p.nsplit * a.k * a.n;
-    a.partD = (T *)work;
-    a.partI = (int *)((char *)work + al16(ents * sizeof(T)));
+    a.partD = (T *)((char *)work + a2b);
+    a.partI = (int *)((char *)work + a2b + al16(ents * sizeof(T)));
   }
   hipError_t e;
-  if (p.kind == 1 && p.direct) {
+  if (a2b > 0) {
+    if constexpr (sizeof(T) == 8) {
+      a.SN = (const T *)work;
+      const int terms = p.ring ? ring_terms(a.d) : 4 * p.ks;
+      const int blocks = std::min((a.m + 3) / 4, 2048);
+      hipLaunchKernelGGL(cand_shift_kernel, dim3(blocks), dim3(256), 0, s, a.m, a.d, terms, a.XA,
+                         a.XA2, a.amap, (double *)work);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+  }
+  if (p.kind == 1 && p.ring) {
+    if constexpr (sizeof(T) == 8) {
+      // 16-byte row loads need d even and both tables 16-byte aligned.
+      const bool vec = a.d % 2 == 0 && ((uintptr_t)a.XA % 16) == 0 && ((uintptr_t)a.XB % 16) == 0;
+      e = launch_knn_ring(a, vec, p.gx, p.lds, s);
+    } else {
+      return hipErrorNotSupported;
+    }
+  } else if (p.kind == 1 && p.direct) {
     if constexpr (sizeof(T) == 8) {
       switch (p.ks) {
         case 4: e = launch_mfma_direct_ks<4>(p, a, s); break;
@@ -1811,7 +1849,7 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
   if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return 0;
   if (elem_bytes == 4 && use_f32_mfma(m, d, k, mode)) return f32_work_bytes(m, n, d, k);
   const Plan p = make_plan(m, n, d, k, elem_bytes, mode);
-  return part_bytes(p, n, k, elem_bytes);
+  return a2_bytes(p, m) + part_bytes(p, n, k, elem_bytes);
 }
 
 bool knn_merge_supported(int k, int elem_bytes) { return merge_qpb(k, elem_bytes) > 0; }

@@ -107,8 +107,10 @@ def test_malloc_aligned():
 
 def test_work_bytes_planning_is_host_only():
     # Split planning: few queries, many candidates -> partial-list workspace.
-    assert gsknn.work_bytes_d(2_000_000, 10, 128, 32) > 0
-    assert gsknn.work_bytes_d(65536, 65536, 64, 32) == 0
+    assert gsknn.work_bytes_d(2_000_000, 10, 128, 32) > 2_000_000 * 8
+    # An unsplit FP64 MFMA launch needs only the candidates' shifted norms
+    # (one double per candidate position).
+    assert gsknn.work_bytes_d(65536, 65536, 64, 32) == 65536 * 8
     assert gsknn.work_bytes_d(0, 10, 8, 4) == 0
 
 
