@@ -1440,7 +1440,8 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   Plan p;
   const bool f64 = elem == 8;
   const int ks = f64 ? mfma_ks(d) : 0;
-  const bool mfma = f64 && ks > 0 &&
+  // Above d = 256 only the operand-ring kernel has an MFMA form.
+  const bool mfma = f64 && (ks > 0 || (ring_supported(d) && mode != MODE_MFMA_TILE)) &&
                     (mode == MODE_MFMA || mode == MODE_MFMA_TILE || (mode == MODE_AUTO && d >= 32));
   int tile_c = 0;
   if (mfma) {
@@ -1454,7 +1455,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
       return e ? atoi(e) : -1;
     }();
     int best = 0, best_waves = 0;
-    for (int c = 0; c < kNumMfmaCfg; ++c) {
+    for (int c = 0; c < kNumMfmaCfg && ks > 0; ++c) {
       if (pinned >= 0 && pinned < kNumMfmaCfg && c != pinned) continue;
       const int qb = kMfmaCfg[c][0], w = kMfmaCfg[c][1], nb = kMfmaCfg[c][2];
       const int Q = w * 16 * qb;
@@ -1473,7 +1474,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
         p.lds = b;
       }
     }
-    tile_c = 16 * mfma_cb(ks);
+    tile_c = ks > 0 ? 16 * mfma_cb(ks) : 0;
     // The direct form (d <= 128): one wave of 16 QB queries per workgroup
     // (QB = 2 up to d = 64, 1 above), only the heap rows in LDS.  Taken by
     // MODE_MFMA wherever it fits; by MODE_AUTO when it keeps at least as many

@@ -25,7 +25,12 @@
 // or 2 when KP is not a multiple of 4): right after the MFMAs of pair j read
 // slot j % R, the slot is refilled with pair j + R, of this tile or (across
 // the tile boundary) of the next one, so each load has R pairs of MFMAs to
-// arrive and only R pairs of candidate operands are ever in registers.  The
+// arrive and only R pairs of candidate operands are ever in registers.
+// KP = 0 is the form for any d (used above d = 256, where the query
+// operands no longer fit in registers): the query operands stream through
+// the ring beside the candidate operands (the wave's query rows stay in the
+// caches), the pair loop runs JR = ceil(ceil(d / 8) / R) R positions, and
+// query dims past d are zeroed where they enter the MFMA.  The
 // candidates' norms shifted by the MFMA error margin (a.SN, by position)
 // are read a tile ahead.
 //
@@ -96,6 +101,7 @@ template <int KP, int QB, int CB, int WPE, bool VEC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void knn_mfma_f64_ring_kernel(
     KnnArgs<double> a) {
   static_assert(QB <= 2, "query block fits one bit");
+  constexpr bool QS = KP == 0;  // query operands streamed through the ring
   constexpr int Q = 16 * QB, C = 16 * CB, R = KP % 4 == 0 ? 4 : 2, QR = kRingQ;
   static_assert(KP % R == 0, "ring slots per tile");
   typedef int idv __attribute__((ext_vector_type(CB * 4)));
@@ -113,9 +119,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
   if (lane < Q && qbase + lane < a.n) heap_load(a, hd, hi, Q, lane, qbase + lane, sentinel);
 
-  const double cm = mfma_margin_c(8 * KP, d);
+  const int JR = QS ? ((d + 7) / 8 + R - 1) / R * R : KP;  // pair positions per tile
+  const double cm = mfma_margin_c(8 * JR, d);
 
-  double bq[QB][2 * KP];
+  double bq[QB][QS ? 1 : 2 * KP];  // register-resident query operands (KP > 0)
+  double qb2[QB];                  // squared norms of the query rows (KP = 0)
+  const double *qp[QB];            // query rows
   double qn[QB];
   int ql[QB];
   bool qv[QB];
@@ -126,12 +135,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     qv[qb] = j < a.n;
     const int row = a.bmap[qv[qb] ? j : a.n - 1];
     const double *xq = a.XB + (size_t)row * d;
+    qp[qb] = xq;
     qn[qb] = qv[qb] ? a.XB2[row] : 0.0;
+    if constexpr (QS) {
+      // Each lane sums the whole row (once per launch).
+      double b2 = 0.0;
+      for (int p = 0; p < d; ++p) b2 = __builtin_fma(xq[p], xq[p], b2);
+      qb2[qb] = qv[qb] ? b2 : 0.0;
+    } else {
 #pragma unroll
-    for (int s = 0; s < 2 * KP; ++s) {
-      const int p = 8 * (s >> 1) + 2 * grp + (s & 1);
-      const double v = xq[p < d ? p : d - 1];
-      bq[qb][s] = (qv[qb] && p < d) ? v : 0.0;
+      for (int s = 0; s < 2 * KP; ++s) {
+        const int p = 8 * (s >> 1) + 2 * grp + (s & 1);
+        const double v = xq[p < d ? p : d - 1];
+        bq[qb][s] = (qv[qb] && p < d) ? v : 0.0;
+      }
     }
   }
   // XB row of query block qb.
@@ -165,6 +182,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   auto row_of = [&](int id) -> const double * { return a.XA + (size_t)id * d; };
 
   rdbl2 xr[R][CB];  // candidate operand ring
+  rdbl2 yr[R][QS ? QB : 1];  // query operand ring (KP = 0)
   double cn[CB];    // shifted norms of this lane's candidates in the current tile (NaN: empty slot)
   auto pos_sn = [&](int t, int cb) -> double { return a.SN[min(cbeg + t * C + cb * 16 + cl, cend - 1)]; };
   int ci[CB];       // their ids (INT_MAX for empty slots)
@@ -190,6 +208,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       cn[cb] = ok ? pos_sn(0, cb) : nan;
       ci[cb] = ok ? id_t.v[cb] : INT_MAX;
     }
+    if constexpr (QS) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+        for (int r = 0; r < R; ++r) yr[r][qb] = load_pair(qp[qb], r);
+    }
     id_a = ntiles > 1 ? cand_id(1) : id_t;
     id_b = ntiles > 2 ? cand_id(2) : id_a;
   }
@@ -200,10 +224,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   // disjoint dims).  Every lane of the wave calls it.
   auto root_thr = [&](int qb) -> double {
     double b2 = 0.0;
+    if constexpr (QS) {
+      b2 = qb2[qb];
+    } else {
 #pragma unroll
-    for (int s = 0; s < 2 * KP; ++s) b2 = __builtin_fma(bq[qb][s], bq[qb][s], b2);
-    b2 = xadd(b2, __shfl_xor(b2, 16));
-    b2 = xadd(b2, __shfl_xor(b2, 32));
+      for (int s = 0; s < 2 * KP; ++s) b2 = __builtin_fma(bq[qb][s], bq[qb][s], b2);
+      b2 = xadd(b2, __shfl_xor(b2, 16));
+      b2 = xadd(b2, __shfl_xor(b2, 32));
+    }
     if (!qv[qb]) return -__builtin_inf();
     return mfma_threshold(hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
   };
@@ -273,25 +301,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = rdbl4{0.0, 0.0, 0.0, 0.0};
 
     __builtin_amdgcn_s_setprio(1);
+    if constexpr (!QS) {
 #pragma unroll
-    for (int j = 0; j < KP; ++j) {
-      const int slot = j % R;
+      for (int j = 0; j < KP; ++j) {
+        const int slot = j % R;
 #pragma unroll
-      for (int e = 0; e < 2; ++e)
+        for (int e = 0; e < 2; ++e)
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-          const double av = e ? xr[slot][cb].y : xr[slot][cb].x;
+          for (int cb = 0; cb < CB; ++cb) {
+            const double av = e ? xr[slot][cb].y : xr[slot][cb].x;
 #pragma unroll
-          for (int qb = 0; qb < QB; ++qb)
-            acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][2 * j + e], acc[qb][cb], 0, 0, 0);
+            for (int qb = 0; qb < QB; ++qb)
+              acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[qb][2 * j + e], acc[qb][cb], 0, 0, 0);
+          }
+        // Refill the slot with pair j + R of this tile, or across the tile
+        // boundary with the next tile's pair j + R - KP (on the last tile the
+        // refill reads valid rows and is never used).
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          xr[slot][cb] = j + R < KP ? load_pair(rc[cb], j + R) : load_pair(rn[cb], j + R - KP);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      for (int j0 = 0; j0 < JR; j0 += R) {
+#pragma unroll
+        for (int slot = 0; slot < R; ++slot) {
+          const int j = j0 + slot;
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            // Query dims past d enter as zero.
+            const bool in = 8 * j + 2 * grp + e < d;
+            double bv[QB];
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) bv[qb] = in ? (e ? yr[slot][qb].y : yr[slot][qb].x) : 0.0;
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+              const double av = e ? xr[slot][cb].y : xr[slot][cb].x;
+#pragma unroll
+              for (int qb = 0; qb < QB; ++qb)
+                acc[qb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[qb], acc[qb][cb], 0, 0, 0);
+            }
+          }
+          // Refill with pair j + R of this tile or the next tile's pair
+          // j + R - JR (the query rows stay the same).
+          const bool nxt = j + R >= JR;
+          const int jn = nxt ? j + R - JR : j + R;
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb) xr[slot][cb] = load_pair(nxt ? rn[cb] : rc[cb], jn);
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) yr[slot][qb] = load_pair(qp[qb], jn);
+          __builtin_amdgcn_sched_barrier(0);
         }
-      // Refill the slot with pair j + R of this tile, or across the tile
-      // boundary with the next tile's pair j + R - KP (on the last tile the
-      // refill reads valid rows and is never used).
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb)
-        xr[slot][cb] = j + R < KP ? load_pair(rc[cb], j + R) : load_pair(rn[cb], j + R - KP);
-      __builtin_amdgcn_sched_barrier(0);
+      }
     }
     __builtin_amdgcn_s_setprio(0);
 
@@ -392,6 +453,9 @@ constexpr int kRingCfg[][4] = {{2, 2, 4, 1},   {4, 2, 4, 1},   {6, 2, 4, 1},   {
                                {20, 2, 2, 1},  {24, 2, 2, 1},  {28, 2, 1, 1},  {32, 2, 1, 1}};
 constexpr int kNumRingCfg = 12;
 
+// Above d = 256: queries streamed (KP = 0), 2 x 2 blocks.
+constexpr int kRingStreamCfg[4] = {0, 2, 2, 1};
+
 bool ring_cfg(int d, RingCfg *c) {
   if (d <= 0) return false;
   for (int i = 0; i < kNumRingCfg; ++i)
@@ -399,7 +463,8 @@ bool ring_cfg(int d, RingCfg *c) {
       *c = {kRingCfg[i][0], kRingCfg[i][1], kRingCfg[i][2], kRingCfg[i][3]};
       return true;
     }
-  return false;
+  *c = {kRingStreamCfg[0], kRingStreamCfg[1], kRingStreamCfg[2], kRingStreamCfg[3]};
+  return true;
 }
 
 template <int KP, int QB, int CB, int WPE, bool VEC>
@@ -425,13 +490,15 @@ int ring_reg_blocks_t() {
   X(20, 2, 2, 1)             \
   X(24, 2, 2, 1)             \
   X(28, 2, 1, 1)             \
-  X(32, 2, 1, 1)
+  X(32, 2, 1, 1)             \
+  X(0, 2, 2, 1)
 
 }  // namespace
 
 int ring_terms(int d) {
   RingCfg c;
-  return ring_cfg(d, &c) ? 8 * c.kp : 0;
+  if (!ring_cfg(d, &c)) return 0;
+  return c.kp > 0 ? 8 * c.kp : 8 * (((d + 7) / 8 + 3) / 4 * 4);
 }
 
 bool ring_supported(int d) {
@@ -459,9 +526,9 @@ size_t ring_lds_bytes(int d, int k) {
 int ring_reg_blocks(int d, bool vec) {
   RingCfg c;
   if (!ring_cfg(d, &c)) return 0;
-  static std::atomic<int> cache[2 * kNumRingCfg];
+  static std::atomic<int> cache[2 * kNumRingCfg + 2];
   int ci = 0;
-  while (kRingCfg[ci][0] != c.kp) ++ci;
+  while (ci < kNumRingCfg && kRingCfg[ci][0] != c.kp) ++ci;
   const int ki = ci * 2 + (vec ? 1 : 0);
   int b = cache[ki].load(std::memory_order_relaxed);
   if (b > 0) return b - 1;
