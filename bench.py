@@ -285,7 +285,7 @@ def precision_form(torch, dev, args):
     out["sample_rows"] = int(rows.size)
     out["sample_bruteforce_ok_bf16"] = ok
     out["bf16_max_rel_dist_err_vs_fp64_bruteforce"] = worst
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def allnn_sweep(torch, dev, args):
@@ -317,7 +317,7 @@ def allnn_sweep(torch, dev, args):
            "graph_replayed_iterations": len(t) - 1,
            "autotuned": {"qpb": sw.config[0], "ct": sw.config[1], "grid": sw.config[2],
                          "tried": (sw.tune_info or {}).get("tried")}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def serving(torch, dev, args):
@@ -364,7 +364,42 @@ def serving(torch, dev, args):
         same += int(np.sum(np.all(np.sort(Ib, 1) == np.sort(If, 1), axis=1)))
         rows += Ib.shape[0]
     out["rows_equal_bf16_vs_fp32"] = same / max(rows, 1)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def north_star_objects(torch, dev, args):
+    """The precision-form, tree-sweep and serving workloads (configs 3, 4 and
+    5 at full size; the serving trace shortened to --extra-requests requests
+    per mode) as compact objects of the default line, each from its own
+    workload function.  A workload that fails is recorded with its error."""
+    import copy
+
+    objs = {}
+    sub = copy.copy(args)
+    sub.scale = 1
+    sub.steps = 3
+    sub.iters = 8
+    sub.requests = args.extra_requests
+    for name, fn, keep in (
+            ("c3", precision_form, ("bf16_rerank", "fp64", "fp32", "speedup_bf16_vs_fp64",
+                                    "rows_equal_bf16_vs_fp64_gpu", "sample_rows",
+                                    "sample_bruteforce_ok_bf16",
+                                    "bf16_max_rel_dist_err_vs_fp64_bruteforce")),
+            ("c4", allnn_sweep, ("N", "iterations", "tree_ms_per_iteration",
+                                 "sweep_ms_per_iteration", "pairs_per_s",
+                                 "graph_replayed_iterations")),
+            ("c5", serving, ("requests", "queries", "rate_req_per_s", "offered_pairs_per_s",
+                             "bf16", "fp32", "rows_equal_bf16_vs_fp32"))):
+        t0 = time.perf_counter()
+        try:
+            full = fn(torch, dev, sub)
+            objs[name] = {k: full[k] for k in keep if k in full}
+        except Exception as e:  # recorded in the line, the headline still prints
+            objs[name] = {"error": f"{type(e).__name__}: {e}"}
+        objs[name]["wall_s"] = time.perf_counter() - t0
+        log(f"{name} object done ({objs[name]['wall_s']:.1f}s)")
+        torch.cuda.empty_cache()
+    return objs
 
 
 def c1_small(torch, dev, reps=200):
@@ -486,6 +521,10 @@ def main():
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
     ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default line without the c3/c4/c5 objects")
+    ap.add_argument("--extra-requests", type=int, default=1024,
+                    help="serving trace length of the default line's c5 object")
     ap.add_argument("--mode", default="auto", choices=["auto", "exact", "mfma", "tile"],
                     help="FP64 contraction (tile: MFMA with the LDS-tile kernel)")
     args = ap.parse_args()
@@ -505,13 +544,13 @@ def main():
         copy_calibration(torch, dev, args)
         return
     if args.workload == "c3":
-        precision_form(torch, dev, args)
+        print(json.dumps(precision_form(torch, dev, args)), flush=True)
         return
     if args.workload == "c4":
-        allnn_sweep(torch, dev, args)
+        print(json.dumps(allnn_sweep(torch, dev, args)), flush=True)
         return
     if args.workload == "c5":
-        serving(torch, dev, args)
+        print(json.dumps(serving(torch, dev, args)), flush=True)
         return
     if args.workload == "l1":
         l1_norm(torch, dev, args)
@@ -647,6 +686,8 @@ def main():
         "rows_ascending": {"ms": sort_ms, "rows": p.n, "k": p.k, "order_ok": sort_ok},
         "c1": c1_small(torch, dev),
     }
+    if not args.no_extras:
+        out.update(north_star_objects(torch, dev, args))
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p)
