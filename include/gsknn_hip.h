@@ -88,6 +88,14 @@ int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
                    int ldk, const int *hrow, int mode, void *work,
                    size_t work_bytes, void *stream);
 
+/* For reporting (no counterpart in the reference): byte offset in the
+ * workspace of a gsknn_sknn_dev call (mode and sizes as there) of one int,
+ * the number of queries whose FP32 MFMA candidate list overflowed its
+ * capacity and were re-scanned against every candidate in the oracle's FP32
+ * arithmetic; (size_t)-1 when such a call does not take the FP32 MFMA form
+ * (d < 8, m < 64, the exact mode, or heaps too large for it). */
+size_t gsknn_sknn_overflow_offset(int m, int n, int d, int k, int mode);
+
 /* Exact kNN from a BF16 MFMA candidate pass and an FP64 re-rank.
  *
  * FP32 point tables XA (NA rows) and XB (NB rows) are first prepared once

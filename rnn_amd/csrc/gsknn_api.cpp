@@ -114,6 +114,11 @@ extern "C" size_t gsknn_sknn_work_bytes(int m, int n, int d, int k, int mode) {
   return knn_work_bytes(m, n, d, k, 4, mode);
 }
 
+extern "C" size_t gsknn_sknn_overflow_offset(int m, int n, int d, int k, int mode) {
+  if (m <= 0 || n <= 0 || d <= 0 || k <= 0 || !knn_f32_takes_mfma(m, d, k, mode)) return (size_t)-1;
+  return f32_overflow_offset(m, n, d, k);
+}
+
 extern "C" int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA, const double *XA2,
                               const int *amap, const double *XB, const double *XB2,
                               const int *bmap, double *D, int *I, int ldk, const int *hrow,

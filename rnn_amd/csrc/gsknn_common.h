@@ -94,7 +94,12 @@ hipError_t launch_rows_ascending_f32(int n, int k, float *D, int *I, int ldk, hi
 // FP32 MFMA candidate pass + re-rank in the oracle's FP32 arithmetic
 // (gsknn_f32.hip).
 size_t f32_work_bytes(int m, int n, int d, int k);
+// Byte offset in that workspace of the count of queries whose candidate list
+// overflowed and were re-scanned ((size_t)-1 without an FP32 MFMA plan).
+size_t f32_overflow_offset(int m, int n, int d, int k);
 bool f32_mfma_supported(int d, int k);
+// Whether an FP32 call (gsknn_kernels.hip) takes that form.
+bool knn_f32_takes_mfma(int m, int d, int k, int mode);
 hipError_t launch_knn_f32_mfma(const KnnArgs<float> &a, hipStream_t s, void *work,
                                size_t work_bytes, int *overflow_out);
 

@@ -657,6 +657,13 @@ size_t f32_work_bytes(int m, int n, int d, int k) {
   return F32Work(m, n, p.cap, p.nsplit).bytes;
 }
 
+size_t f32_overflow_offset(int m, int n, int d, int k) {
+  F32Plan p = plan_f32(d, k);
+  if (p.q == 0 || m <= 0 || n <= 0) return (size_t)-1;
+  plan_f32_splits(p, m, n);
+  return F32Work(m, n, p.cap, p.nsplit).nover;
+}
+
 bool f32_mfma_supported(int d, int k) {
   const F32Plan p = plan_f32(d, k);
   const size_t rl = a16(sizeof(float) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;

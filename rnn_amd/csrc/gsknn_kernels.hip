@@ -1855,6 +1855,8 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
 
 bool knn_merge_supported(int k, int elem_bytes) { return merge_qpb(k, elem_bytes) > 0; }
 
+bool knn_f32_takes_mfma(int m, int d, int k, int mode) { return use_f32_mfma(m, d, k, mode); }
+
 bool knn_merge_query_major(int n, int k, int nsplit) { return merge_query_major(n, k, nsplit); }
 
 hipError_t launch_knn_merge(const KnnArgs<double> &a, hipStream_t s) { return launch_merge(a, s); }

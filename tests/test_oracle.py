@@ -198,3 +198,53 @@ def test_oracle_matches_reference_optimized_path(d):
         np.testing.assert_array_equal(Irs, Ios)
         scale = X2[bmap][:, None] + X2.max()
         assert np.all(np.abs(Drs - Dos) <= 1e-12 * scale)
+
+
+def test_direct_knn_f32in_matches_long_double_brute_force():
+    """The FP64 direct-difference checker of the BF16 form equals long-double
+    brute force on FP32 points: same neighbour sets in (distance, index)
+    order, distances within one rounding of the FP64 sum (the long-double
+    sum rounded to FP64 is the reference); lattice points give exact ties
+    that the lower index must win."""
+    g = np.random.Generator(np.random.PCG64(91))
+    for d, lattice in ((1, False), (7, False), (33, False), (5, True)):
+        if lattice:
+            X = g.integers(-3, 4, (300, d)).astype(np.float32)
+        else:
+            X = g.standard_normal((300, d)).astype(np.float32)
+        amap = g.permutation(300)[:250].astype(np.int32)
+        bmap = g.integers(0, 300, 40).astype(np.int32)
+        k = 12
+        D = np.full((40, k), np.inf)
+        I = np.full((40, k), -1, dtype=np.int32)
+        oracle.direct_knn_f32in(X, amap, X, bmap, D, I)
+        Ds, Is = oracle.sort_rows(D, I)
+        Db, Ib = oracle.brute(X.astype(np.float64), amap, X.astype(np.float64), bmap, k)
+        np.testing.assert_array_equal(Is, Ib)
+        np.testing.assert_allclose(Ds, Db, rtol=4 * d * 2.0 ** -53, atol=0)
+        if lattice:
+            assert np.array_equal(Ds, Db)  # integer coordinates: every sum is exact
+
+
+def test_rows_are_heaps_rejects_a_planted_violation():
+    """rows_are_heaps is a real check: a valid max-heap row passes, and the
+    same row with a child swapped above its parent, or (canonically) an
+    equal-distance child of larger index above its parent, fails."""
+    D = np.array([[9.0, 7.0, 8.0, 1.0, 2.0, 3.0]])
+    I = np.array([[5, 4, 3, 2, 1, 0]], dtype=np.int32)
+    assert oracle.rows_are_heaps(D, I)
+    Dv = D.copy()
+    Dv[0, [0, 1]] = Dv[0, [1, 0]]  # root 7 below its child 9
+    Iv = I.copy()
+    Iv[0, [0, 1]] = Iv[0, [1, 0]]
+    assert not oracle.rows_are_heaps(Dv, Iv)
+    Dv = D.copy()
+    Dv[0, 4] = 7.5  # a grandchild above its parent 7
+    assert not oracle.rows_are_heaps(Dv, I)
+    # Canonical order: parent (7, 4) with an equal-distance child (7, 9).
+    Dt = np.array([[9.0, 7.0, 8.0, 7.0]])
+    It = np.array([[0, 4, 3, 9]], dtype=np.int32)
+    assert oracle.rows_are_heaps(Dt, It)  # distance order only
+    assert not oracle.rows_are_heaps(Dt, It, canonical=True)
+    It[0, 3] = 1  # equal distance, smaller index below: a canonical heap
+    assert oracle.rows_are_heaps(Dt, It, canonical=True)
