@@ -46,21 +46,22 @@ def _orderable(f):
     return b.to(torch.int64) + (1 << 31)
 
 
-def build_tree(X, iteration, seed, leaf_lo=256, leaf_hi=4096):
-    """One tree over the rows of device tensor X (N, d), float32.
+def plan_tree(N, d, iteration, seed, leaf_lo=256, leaf_hi=4096):
+    """The node structure of one tree, on the host alone.
 
-    Returns (perm, leaves): perm an int32 device tensor of point ids grouped
-    by leaf, leaves a list of (start, size) in perm order.  Node bookkeeping
-    is vectorized (numpy on the host, one pass of device work per level).
+    Exact median splits make every node's size a function of its parent's
+    (halves), so the whole tree follows from N and the seeded draws: per
+    level, a stop threshold log-uniform in [leaf_lo, leaf_hi] for each node,
+    then a unit direction for each node that splits.  Returns (levels,
+    leaves): levels a list of (starts, sizes, dirs) for the nodes split at
+    each level (int64, int64, float64 [nodes, d]), leaves a list of (start,
+    size) in perm order.
     """
-    import torch
-    dev = X.device
-    N, d = X.shape
     g = np.random.Generator(np.random.PCG64([seed, iteration]))
-    perm = torch.arange(N, device=dev, dtype=torch.int64)
     n_start = np.zeros(1, dtype=np.int64)
     n_size = np.full(1, N, dtype=np.int64)
     leaf_parts = []
+    levels = []
     while n_start.size:
         thr = np.exp(g.uniform(np.log(leaf_lo), np.log(leaf_hi), size=n_start.size))
         split = (n_size > thr) & (n_size >= 2)
@@ -70,47 +71,89 @@ def build_tree(X, iteration, seed, leaf_lo=256, leaf_hi=4096):
         starts, sizes = n_start[split], n_size[split]
         dirs = g.standard_normal((starts.size, d))
         dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-        dirs_t = torch.from_numpy(dirs.astype(np.float32)).to(dev)
-        starts_t = torch.from_numpy(starts).to(dev)
-        total = int(sizes.sum())
-        # order inside each node by (projection, point id): one sort of the
-        # packed key (node, projection, id) when it fits in 63 bits, else a
-        # sort by id followed by a stable sort by (node, projection)
-        id_bits = max(1, int(N - 1).bit_length())
-        node_bits = max(1, int(starts.size - 1).bit_length())
-        fits = node_bits + 32 + id_bits <= 63
-        if fits and X.is_cuda:
-            # positions and keys from one fused kernel (libgsknn.so)
-            offs_t = torch.from_numpy(np.cumsum(sizes) - sizes).to(dev)
-            pos = torch.empty(total, dtype=torch.int64, device=dev)
-            key = torch.empty(total, dtype=torch.int64, device=dev)
-            gsknn.tree_level_keys_dev(X, perm, starts_t, offs_t, dirs_t, total, id_bits, pos, key)
-            perm[pos] = torch.sort(key).values & ((1 << id_bits) - 1)
-            half = sizes // 2
-            n_start = np.stack([starts, starts + half], axis=1).reshape(-1)
-            n_size = np.stack([half, sizes - half], axis=1).reshape(-1)
-            continue
-        sizes_t = torch.from_numpy(sizes).to(dev)
-        node = torch.repeat_interleave(torch.arange(starts.size, device=dev), sizes_t,
-                                       output_size=total)
-        offs = torch.cumsum(sizes_t, 0) - sizes_t
-        pos = starts_t[node] + (torch.arange(node.numel(), device=dev) - offs[node])
-        pts = perm[pos]
-        proj = (X[pts] * dirs_t[node]).sum(dim=1)
-        if fits:
-            key = (node << (32 + id_bits)) | (_orderable(proj) << id_bits) | pts
-            perm[pos] = pts[torch.argsort(key)]
-        else:
-            o1 = torch.argsort(node * N + pts)
-            node1, pts1, proj1 = node[o1], pts[o1], proj[o1]
-            o2 = torch.sort(node1 * (1 << 33) + _orderable(proj1), stable=True).indices
-            perm[pos] = pts1[o2]
+        levels.append((starts, sizes, dirs))
         half = sizes // 2
         n_start = np.stack([starts, starts + half], axis=1).reshape(-1)
         n_size = np.stack([half, sizes - half], axis=1).reshape(-1)
     leaves = np.concatenate(leaf_parts) if leaf_parts else np.zeros((0, 2), np.int64)
     leaves = leaves[np.argsort(leaves[:, 0], kind="stable")]
-    return perm.to(torch.int32), [(int(s_), int(n_)) for s_, n_ in leaves]
+    return levels, [(int(s_), int(n_)) for s_, n_ in leaves]
+
+
+def build_tree(X, iteration, seed, leaf_lo=256, leaf_hi=4096, perm_out=None, plan=None):
+    """One tree over the rows of device tensor X (N, d), float32.
+
+    Returns (perm, leaves): perm an int32 device tensor of point ids grouped
+    by leaf (written into perm_out when given), leaves a list of (start,
+    size) in perm order.  The node structure comes from plan_tree on the
+    host (or `plan`, its result); the per-level node arrays go to the device
+    in one copy, and each level is one fused key kernel (libgsknn.so) and one
+    sort of the keys, on the current stream with no host synchronisation.
+    """
+    import torch
+    dev = X.device
+    N, d = X.shape
+    levels, leaves = plan if plan is not None else plan_tree(N, d, iteration, seed, leaf_lo, leaf_hi)
+    perm = torch.arange(N, device=dev, dtype=torch.int64)
+    id_bits = max(1, int(N - 1).bit_length())
+    fused = X.is_cuda and all(max(1, int(st.size - 1).bit_length()) + 32 + id_bits <= 63
+                              for st, _, _ in levels)
+    if levels and fused:
+        # One upload: per level, starts and element offsets (int64), then
+        # the directions (float32).
+        nodes = np.array([st.size for st, _, _ in levels], dtype=np.int64)
+        node_off = np.concatenate([[0], np.cumsum(nodes)])
+        ints = np.concatenate([np.concatenate([st, np.cumsum(sz) - sz]) for st, sz, _ in levels])
+        dirs = np.concatenate([dr for _, _, dr in levels]).astype(np.float32)
+        ints_t = torch.from_numpy(ints).pin_memory().to(dev, non_blocking=True)
+        dirs_t = torch.from_numpy(dirs).pin_memory().to(dev, non_blocking=True)
+        int_off = 0
+        for li, (starts, sizes, _) in enumerate(levels):
+            nn = starts.size
+            starts_t = ints_t[int_off:int_off + nn]
+            offs_t = ints_t[int_off + nn:int_off + 2 * nn]
+            int_off += 2 * nn
+            total = int(sizes.sum())
+            pos = torch.empty(total, dtype=torch.int64, device=dev)
+            key = torch.empty(total, dtype=torch.int64, device=dev)
+            gsknn.tree_level_keys_dev(X, perm, starts_t, offs_t,
+                                      dirs_t[node_off[li]:node_off[li + 1]], total, id_bits,
+                                      pos, key)
+            perm[pos] = torch.sort(key).values & ((1 << id_bits) - 1)
+    else:
+        for starts, sizes, dirs in levels:
+            _split_level_torch(X, perm, starts, sizes, dirs, id_bits)
+    if perm_out is not None:
+        perm_out.copy_(perm)
+        return perm_out, leaves
+    return perm.to(torch.int32), leaves
+
+
+def _split_level_torch(X, perm, starts, sizes, dirs, id_bits):
+    """One level with torch operations (CPU tensors, or node counts too
+    large for the packed key): order each split node by (projection, id)."""
+    import torch
+    dev = X.device
+    N = X.shape[0]
+    dirs_t = torch.from_numpy(dirs.astype(np.float32)).to(dev)
+    starts_t = torch.from_numpy(starts).to(dev)
+    total = int(sizes.sum())
+    sizes_t = torch.from_numpy(sizes).to(dev)
+    node = torch.repeat_interleave(torch.arange(starts.size, device=dev), sizes_t,
+                                   output_size=total)
+    offs = torch.cumsum(sizes_t, 0) - sizes_t
+    pos = starts_t[node] + (torch.arange(node.numel(), device=dev) - offs[node])
+    pts = perm[pos]
+    proj = (X[pts] * dirs_t[node]).sum(dim=1)
+    node_bits = max(1, int(starts.size - 1).bit_length())
+    if node_bits + 32 + id_bits <= 63:
+        key = (node << (32 + id_bits)) | (_orderable(proj) << id_bits) | pts
+        perm[pos] = pts[torch.argsort(key)]
+    else:
+        o1 = torch.argsort(node * N + pts)
+        node1, pts1, proj1 = node[o1], pts[o1], proj[o1]
+        o2 = torch.sort(node1 * (1 << 33) + _orderable(proj1), stable=True).indices
+        perm[pos] = pts1[o2]
 
 
 def make_items(leaves, qpb):
@@ -197,10 +240,17 @@ def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None):
 
 
 class RPTreeSweep:
-    """All-NN sweep state: points, global heap rows, the captured launch."""
+    """All-NN sweep state: points, global heap rows, the captured launch.
+
+    step() runs one iteration.  With prefetch (the default on a GPU) the
+    next iteration's tree is planned on the host and built on a second
+    stream while the current sweep runs, into one of two spare permutation
+    buffers, so an iteration costs about the larger of the two instead of
+    their sum.
+    """
 
     def __init__(self, X, k, seed=3, leaf_lo=256, leaf_hi=4096, dtype="f32", config=None,
-                 use_graph=True, log=None):
+                 use_graph=True, log=None, prefetch=True):
         import torch
         self.X = X
         self.k = k
@@ -222,6 +272,33 @@ class RPTreeSweep:
         self.tune_info = None
         self.iteration = 0
         self.timings = []
+        self.prefetch = prefetch and X.is_cuda
+        self._bufs = [torch.empty(N, dtype=torch.int32, device=dev) for _ in range(2)]
+        self._side = torch.cuda.Stream(device=dev) if self.prefetch else None
+        self._pending = None
+        self._last_end = None
+
+    def _build(self, it, after=None):
+        """Build tree `it` into spare buffer it % 2; on the side stream with
+        prefetch (after event `after`: the buffer's last reader is done).
+        Returns (perm buffer, leaves, start event, done event)."""
+        import torch
+        buf = self._bufs[it % 2]
+        stream = self._side if self.prefetch else torch.cuda.current_stream()
+        t0 = time.perf_counter()
+        plan = plan_tree(self.X.shape[0], self.X.shape[1], it, self.seed, self.leaf_lo,
+                         self.leaf_hi)
+        self._plan_ms = (time.perf_counter() - t0) * 1e3
+        with torch.cuda.stream(stream):
+            if after is not None:
+                stream.wait_event(after)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _, leaves = build_tree(self.X, it, self.seed, self.leaf_lo, self.leaf_hi, perm_out=buf,
+                                   plan=plan)
+            e1.record(stream)
+        return buf, leaves, e0, e1
 
     def _ensure_config(self, perm, leaves):
         if self.config is None:
@@ -243,32 +320,44 @@ class RPTreeSweep:
                 self.nitems, self.D, self.I, qpb, ct, grid, stream)
 
     def step(self):
-        """One tree iteration: build the tree, rewrite the work list, launch."""
+        """One tree iteration: (wait for / build) its tree, rewrite the work
+        list, launch the sweep; with prefetch, start the next tree."""
         import torch
+        main = torch.cuda.current_stream()
         t0 = time.perf_counter()
-        perm, leaves = build_tree(self.X, self.iteration, self.seed, self.leaf_lo, self.leaf_hi)
-        self._ensure_config(perm, leaves)
+        if self._pending is None:
+            self._pending = self._build(self.iteration)
+        buf, leaves, et0, et1 = self._pending
+        self._pending = None
+        if self.config is None:
+            torch.cuda.synchronize()  # the autotuner times launches on this tree
+        self._ensure_config(buf, leaves)
         qpb = self.config[0]
-        items = make_items(leaves, qpb)
-        self.perm.copy_(perm)
-        self.items[: items.shape[0]].copy_(torch.from_numpy(items))
+        items = torch.from_numpy(make_items(leaves, qpb)).pin_memory()
+        main.wait_event(et1)
+        self.perm.copy_(buf)
+        copied = torch.cuda.Event()
+        copied.record(main)
+        self.items[: items.shape[0]].copy_(items, non_blocking=True)
         self.nitems.fill_(items.shape[0])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        stream = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(main)
         if self.use_graph and self.graph is not None:
-            e0.record(stream)
             self.graph.replay()
-            e1.record(stream)
         else:
             # First iteration (or no graph): launch directly; the direct
             # launch also sets the kernel's attributes before any capture.
-            e0.record(stream)
             self._launch_captured()
-            e1.record(stream)
+        e1.record(main)
+        if self.prefetch:
+            # The next tree goes to the other buffer (read by the previous
+            # iteration's copy, long finished); it waits for this copy only
+            # so that the two streams never touch one buffer at once.
+            self._pending = self._build(self.iteration + 1, after=copied)
         torch.cuda.synchronize()
+        t1 = time.perf_counter()
         sweep_ms = e0.elapsed_time(e1)
+        tree_ms = et0.elapsed_time(et1)
         if self.use_graph and self.graph is None:
             # Capture the launch once; later iterations rewrite perm/items/
             # nitems in place and replay it.  Capturing does not execute.
@@ -277,9 +366,10 @@ class RPTreeSweep:
                 self._launch_captured()
             torch.cuda.synchronize()
         pairs = float(sum(n * n for _, n in leaves))
-        info = {"iteration": self.iteration, "tree_ms": (t1 - t0) * 1e3, "sweep_ms": sweep_ms,
-                "leaves": len(leaves), "pairs": pairs,
+        info = {"iteration": self.iteration, "tree_ms": tree_ms, "sweep_ms": sweep_ms,
+                "tree_plan_ms": self._plan_ms,
+                "iteration_ms": (t1 - t0) * 1e3, "leaves": len(leaves), "pairs": pairs,
                 "leaf_sizes": sorted({n for _, n in leaves})}
         self.timings.append(info)
         self.iteration += 1
-        return perm, leaves, info
+        return buf, leaves, info
