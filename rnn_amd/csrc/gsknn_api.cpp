@@ -13,6 +13,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -41,7 +43,7 @@ static int hip_err(hipError_t e, const char *what) {
 
 extern "C" const char *gsknn_last_error(void) { return g_err; }
 
-static int g_device = 0;
+static std::atomic<int> g_device{0};
 
 extern "C" int gsknn_set_device(int device) {
   int count = 0;
@@ -360,15 +362,21 @@ bool is_device_ptr(const void *p) {
   return at.type == hipMemoryTypeDevice;
 }
 
-// Grow-only device buffers shared by the host-pointer entry points, which
-// serialise on g_stage_mu.
+// Grow-only device buffers and a stream per device, for the host-pointer
+// entry points, which serialise on g_stage_mu.  begin_call points g_buf at
+// the buffers of the device selected by gsknn_set_device, so a device switch
+// between calls never reuses another device's stream or memory.
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
 };
+struct DevState {
+  DevBuf buf[10];
+  hipStream_t stream = nullptr;
+};
 std::mutex g_stage_mu;
-DevBuf g_buf[10];
-hipStream_t g_stream = nullptr;
+std::map<int, DevState> g_state;
+DevBuf *g_buf = nullptr;
 
 int ensure(DevBuf &b, size_t bytes, void **out) {
   if (bytes == 0) bytes = 16;
@@ -465,14 +473,18 @@ int stage_inputs(const char *fn, int m, int n, int d, const T *XA, const T *XA2,
   return GSKNN_OK;
 }
 
+// Caller holds g_stage_mu.
 int begin_call(hipStream_t *s) {
-  hipError_t e = hipSetDevice(g_device);
+  const int dev = g_device.load();
+  hipError_t e = hipSetDevice(dev);
   if (e != hipSuccess) return hip_err(e, "hipSetDevice");
-  if (!g_stream) {
-    e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+  DevState &st = g_state[dev];
+  if (!st.stream) {
+    e = hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_err(e, "hipStreamCreate");
   }
-  *s = g_stream;
+  g_buf = st.buf;
+  *s = st.stream;
   return GSKNN_OK;
 }
 

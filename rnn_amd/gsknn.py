@@ -67,6 +67,8 @@ def lib() -> ctypes.CDLL:
 def _declare(L):
     vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     L.gsknn_last_error.restype = ctypes.c_char_p
+    L.gsknn_set_device.argtypes = [ctypes.c_int]
+    L.gsknn_set_device.restype = ctypes.c_int
     L.gsknn_dknn_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp, sz, vp]
     L.gsknn_dknn_dev.restype = i32
     L.gsknn_dknn_noalloc_dev.argtypes = [i32] * 4 + [vp] * 8 + [i32, vp, i32, vp]

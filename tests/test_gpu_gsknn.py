@@ -102,6 +102,28 @@ def test_config1_full_both_calls():
         assert_rows(D, I, Dr, Ir)
 
 
+def test_set_device_binds_host_api_to_that_device():
+    # gsknn_set_device rejects devices that do not exist and leaves the
+    # selection unchanged; the host-pointer API then stages and launches on
+    # the selected device's own stream and buffers.
+    import torch
+    L = gsknn.lib()
+    count = torch.cuda.device_count()
+    assert L.gsknn_set_device(-1) == 1
+    assert L.gsknn_set_device(count) == 1
+    assert "gsknn_set_device" in gsknn.last_error()
+    p = ds.config1()
+    for dev in range(count):
+        assert L.gsknn_set_device(dev) == 0
+        print(f"[gpu] set_device {dev} config1 m=n=2048 d=16 k=16", flush=True)
+        D, I = ds.empty_heap(p.n, p.k)
+        Dr, Ir = D.copy(), I.copy()
+        gsknn.dgsknn_heap(p.XA, p.XA2, p.amap, p.XB, p.XB2, p.bmap, D, I)
+        oracle.gsknn(p.XA, p.XA2, p.amap, p.XB, p.XB2, p.bmap, Dr, Ir)
+        assert_rows(D, I, Dr, Ir)
+    assert L.gsknn_set_device(0) == 0
+
+
 def test_config1_literal_second_call():
     # The other reading of the second call: a disjoint query set into the
     # same heap rows.
