@@ -96,6 +96,15 @@ int gsknn_sknn_dev(int m, int n, int d, int k, const float *XA,
  * (d < 8, m < 64, the exact mode, or heaps too large for it). */
 size_t gsknn_sknn_overflow_offset(int m, int n, int d, int k, int mode);
 
+/* For reporting (no counterpart in the reference): byte offset in the
+ * workspace of a gsknn_dknn_dev call (mode and sizes as there, workspace of
+ * gsknn_dknn_work_bytes bytes) of one int, the number of queries the FP64
+ * MFMA two-stage form could not settle from its key heaps (more than the
+ * slack of candidates within the MFMA error margin of the k-th distance)
+ * and re-ran in one stage; (size_t)-1 when such a call does not take the
+ * two-stage form (exact mode, candidate splits, LDS-tile kernel). */
+size_t gsknn_dknn_fallback_offset(int m, int n, int d, int k, int mode);
+
 /* Exact kNN from a BF16 MFMA candidate pass and an FP64 re-rank.
  *
  * FP32 point tables XA (NA rows) and XB (NB rows) are first prepared once

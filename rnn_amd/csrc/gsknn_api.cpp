@@ -78,14 +78,22 @@ static int knn_dev(const char *fn, int m, int n, int d, int k, const T *XA, cons
   if (!XA || !XA2 || !amap || !XB || !XB2 || !bmap || !D || !I)
     return set_err(GSKNN_EINVAL, "%s: null array", fn);
   hipStream_t s = (hipStream_t)stream;
-  const size_t need = knn_work_bytes(m, n, d, k, (int)sizeof(T), mode);
+  size_t need = knn_work_bytes(m, n, d, k, (int)sizeof(T), mode);
   void *own = nullptr;
-  if (!allow_alloc && need > work_bytes)
-    return set_err(GSKNN_ELIMIT, "%s: the call needs %zu bytes of scratch, more than the %zu available",
-                   fn, need, work_bytes);
-  if (need > 0 && (work == nullptr || work_bytes < need)) {
-    if (work != nullptr)
-      return set_err(GSKNN_EINVAL, "%s: workspace %zu < %zu bytes", fn, work_bytes, need);
+  if (!allow_alloc && need > work_bytes) {
+    need = knn_work_bytes_min(m, n, d, k, (int)sizeof(T), mode);
+    if (need > work_bytes)
+      return set_err(GSKNN_ELIMIT, "%s: the call needs %zu bytes of scratch, more than the %zu available",
+                     fn, need, work_bytes);
+  }
+  // A caller's buffer of at least the least workspace is used as it is (the
+  // FP64 MFMA per-wave forms then run in one stage).
+  if (need > 0 && work != nullptr && work_bytes < need) {
+    const size_t least = knn_work_bytes_min(m, n, d, k, (int)sizeof(T), mode);
+    if (work_bytes < least)
+      return set_err(GSKNN_EINVAL, "%s: workspace %zu < %zu bytes", fn, work_bytes, least);
+  }
+  if (need > 0 && work == nullptr) {
     hipError_t e = hipMallocAsync(&own, need, s);
     if (e != hipSuccess) return hip_err(e, "hipMallocAsync(workspace)");
     work = own;
@@ -119,6 +127,10 @@ extern "C" size_t gsknn_sknn_work_bytes(int m, int n, int d, int k, int mode) {
 extern "C" size_t gsknn_sknn_overflow_offset(int m, int n, int d, int k, int mode) {
   if (m <= 0 || n <= 0 || d <= 0 || k <= 0 || !knn_f32_takes_mfma(m, d, k, mode)) return (size_t)-1;
   return f32_overflow_offset(m, n, d, k);
+}
+
+extern "C" size_t gsknn_dknn_fallback_offset(int m, int n, int d, int k, int mode) {
+  return knn_two_stage_count_offset(m, n, d, k, mode);
 }
 
 extern "C" int gsknn_dknn_dev(int m, int n, int d, int k, const double *XA, const double *XA2,

@@ -31,6 +31,16 @@ struct KnnArgs {
   const T *SN;      // FP64 MFMA: candidate position i's norm XA2[amap[i]]
                     // shifted down by the MFMA error margin (mfma_shift_norm,
                     // from the row's squared norm computed by the library)
+  // FP64 MFMA two-stage form (unsplit launches): the streaming kernel keeps
+  // the k2 = k + slack smallest candidates of each query under their MFMA
+  // key, [n][k2] in hk/hid (heap order, root first), and the finalize kernel
+  // forms their oracle distances.
+  int k2;
+  T *hk;
+  int *hid;
+  // Launches over a device-side query list: when set, the query count is
+  // *ncount (a.n only bounds it), read by the kernel.
+  const int *ncount;
 };
 
 enum Mode {
@@ -45,8 +55,14 @@ hipError_t launch_knn_f64(const KnnArgs<double> &a, int mode, hipStream_t s,
                           void *work, size_t work_bytes);
 hipError_t launch_knn_f32(const KnnArgs<float> &a, int mode, hipStream_t s,
                           void *work, size_t work_bytes);
-// Workspace bytes the launch above needs (partial lists of split launches).
+// Workspace bytes the launch above needs (shifted norms, partial lists of
+// split launches, the two-stage form's key heaps), and the least it can run
+// with (the FP64 MFMA per-wave forms then run in one stage).
 size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
+size_t knn_work_bytes_min(int m, int n, int d, int k, int elem_bytes, int mode);
+// Byte offset in an FP64 call's workspace of the two-stage form's fallback
+// count ((size_t)-1 without that form).
+size_t knn_two_stage_count_offset(int m, int n, int d, int k, int mode);
 
 // FP64 MFMA operand-ring kernel (gsknn_ring.hip), d <= 256: queries per
 // workgroup, candidates per tile, LDS bytes, workgroups per CU that its
@@ -55,10 +71,11 @@ size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode);
 bool ring_supported(int d);
 int ring_queries_per_block(int d);
 int ring_candidates_per_tile(int d);
-size_t ring_lds_bytes(int d, int k);
+size_t ring_lds_bytes(int d, int k, bool two);
 int ring_reg_blocks(int d, bool vec);
 int ring_terms(int d);  // MFMA terms per dot product (8 x pairs)
-hipError_t launch_knn_ring(const KnnArgs<double> &a, bool vec, int gx, size_t lds, hipStream_t s);
+hipError_t launch_knn_ring(const KnnArgs<double> &a, bool vec, int gx, size_t lds, hipStream_t s,
+                           bool two);
 
 // Merge of split partial lists into the heap rows (gsknn_kernels.hip);
 // supported when a workgroup's heaps fit in LDS.

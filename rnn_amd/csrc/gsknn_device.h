@@ -88,6 +88,90 @@ __device__ __forceinline__ double mfma_threshold(double r, double qn, double hal
   return xadd(xsub(r, qn), m);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 MFMA two-stage form.  Stage 1 (the streaming kernel) keeps, per query,
+// the k2 = k + slack smallest candidates under the canonical order of
+// (t, id), t = fl(-2 s_m + shifted cn) the pair's MFMA key, in an LDS
+// max-heap (no row is read for a passing pair), and writes it out with its
+// root first.  A candidate leaves the heap, or never enters it, only with a
+// key at or above the heap's root at that moment, and roots only fall, so
+// every candidate outside the final heap has t >= T, the final root key
+// (when the heap never filled, no candidate was left out by it).
+// Candidates the filter drops under the resident row's root r0 have
+// t > mfma_threshold(r0), so their oracle distance exceeds r0.
+// Stage 2 (knn_two_stage_finalize_kernel) forms the oracle distances of the
+// heap's entries, merges them with the resident row and takes the k smallest
+// entries; E, the k-th, is then at most r0.  If the heap filled and
+// T > mfma_threshold(E), every candidate left out has t > mfma_threshold(E)
+// and so an oracle distance above E: the row is the oracle's.  Otherwise the
+// query is re-run by the one-stage kernel (oracle distance for every passing
+// pair), which happens only when more than `slack` candidates lie within the
+// MFMA margin of the k-th distance (exact ties, duplicated points).
+// ---------------------------------------------------------------------------
+struct MfmaHeapLds {
+  size_t hd, hi, r0, bytes;
+  __host__ __device__ MfmaHeapLds(int Q, int kh, bool two) {
+    hd = 0;
+    hi = al16(sizeof(double) * (size_t)kh * Q);
+    r0 = al16(hi + sizeof(int) * (size_t)kh * Q);
+    bytes = two ? r0 + sizeof(double) * (size_t)Q : hi + sizeof(int) * (size_t)kh * Q;
+  }
+};
+
+// Heap entries per query of the two-stage form.
+__host__ __device__ constexpr int two_stage_k2(int k) { return k + (k / 8 > 8 ? k / 8 : 8); }
+
+// Entries the finalize kernel sorts per query: the next power of two
+// >= k + k2 (at least 64); the form is planned only up to kTwoMaxP.
+__host__ __device__ inline int two_stage_sort_size(int k, int k2) {
+  int P = 64;
+  while (P < k + k2) P *= 2;
+  return P;
+}
+constexpr int kTwoMaxP = 2048;
+
+// Empty key heap of slot q (kh entries of (+inf, INT_MAX)); r0[q] = the
+// resident row's largest distance.
+template <typename T>
+__device__ void key_heap_init(const KnnArgs<T> &a, T *hd, int *hi, T *r0, int QS, int q, int j,
+                              int kh) {
+  for (int t = 0; t < kh; ++t) {
+    hd[t * QS + q] = __builtin_inf();
+    hi[t * QS + q] = INT_MAX;
+  }
+  const T *Dr = a.D + (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  const int *Ir = a.I + (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  T md = Dr[0];
+  int mi = Ir[0];
+  for (int t = 1; t < a.k; ++t)
+    if (cless(md, mi, Dr[t], Ir[t])) {
+      md = Dr[t];
+      mi = Ir[t];
+    }
+  r0[q] = md;
+}
+
+// Filter threshold of the two-stage form: the resident row's threshold, or
+// the key heap's root once it is below it.  A NaN r0 admits nothing, as in
+// the one-stage form.
+__device__ __forceinline__ double key_threshold(double r0, double hroot, double qn,
+                                                double half_c_b2) {
+  const double t0 = mfma_threshold(r0, qn, half_c_b2);
+  return hroot < t0 ? hroot : t0;
+}
+
+// Key heap of slot q out to row j of a.hk/a.hid, heap order (root first).
+template <typename T>
+__device__ void key_heap_store(const KnnArgs<T> &a, const T *hd, const int *hi, int QS, int q,
+                               int j, int kh) {
+  T *ok = a.hk + (size_t)j * kh;
+  int *oi = a.hid + (size_t)j * kh;
+  for (int t = 0; t < kh; ++t) {
+    ok[t] = hd[t * QS + q];
+    oi[t] = hi[t * QS + q];
+  }
+}
+
 // Constant-address-space view of a global array: uniform reads through it
 // become scalar loads (SGPR operands shared by the whole wave).  Only arrays
 // a kernel never writes are viewed so.

@@ -509,7 +509,7 @@ __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(KS, QB, CB, W)))) void knn_
 // knn_mfma_f64_kernel; the candidate norms and ids of a lane's result values
 // come from the lanes that loaded them, by lane shuffles.
 // ---------------------------------------------------------------------------
-template <int KS, int QB, int CB, int W, bool LINE>
+template <int KS, int QB, int CB, int W, bool LINE, bool TWO>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void knn_mfma_f64_direct_kernel(
     KnnArgs<double> a) {
   static_assert(KS % 4 == 0, "k-steps in groups of four");
@@ -518,18 +518,28 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   typedef int idv __attribute__((ext_vector_type(CB * 4)));
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k, d = a.d;
-  double *hd = (double *)smem;
-  int *hi = (int *)(smem + al16(sizeof(double) * (size_t)k * Q));
+  const int n = a.ncount ? *a.ncount : a.n;
+  const int qbase = blockIdx.x * Q;
+  if (qbase >= n) return;  // fallback launches over a device-side list
+  const int kh = TWO ? a.k2 : k;  // heap entries per query
+  const MfmaHeapLds hl(Q, kh, TWO);
+  double *hd = (double *)(smem + hl.hd);
+  int *hi = (int *)(smem + hl.hi);
+  double *r0 = (double *)(smem + hl.r0);  // two-stage: roots of the resident rows
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, grp = lane >> 4, cl = lane & 15;
-  const int qbase = blockIdx.x * Q;
   const int split = blockIdx.y;
   const int cbeg = split * a.cps;
   const int cend = min(a.m, cbeg + a.cps);
   const bool sentinel = a.nsplit > 1;
 
-  if (tid < Q && qbase + tid < a.n) heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
+  if (tid < Q && qbase + tid < n) {
+    if (TWO)
+      key_heap_init(a, hd, hi, r0, Q, tid, qbase + tid, kh);
+    else
+      heap_load(a, hd, hi, Q, tid, qbase + tid, sentinel);
+  }
 
   const int own0 = grp * KS, own1 = min(own0 + KS, d);  // dims this lane's group owns
   const int st = LINE ? 2 * grp : min(own0, d - KS);       // first dim it reads
@@ -544,8 +554,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   for (int qb = 0; qb < QB; ++qb) {
     ql[qb] = w * QW + qb * 16 + cl;
     const int j = qbase + ql[qb];
-    qv[qb] = j < a.n;
-    const int jj = qv[qb] ? j : a.n - 1;
+    qv[qb] = j < n;
+    const int jj = qv[qb] ? j : n - 1;
     const double *xq = a.XB + (size_t)a.bmap[jj] * d;
     qn[qb] = qv[qb] ? a.XB2[a.bmap[jj]] : 0.0;
 #pragma unroll
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
   }
   // XB row of query block qb (every lane of the wave calls it).
-  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : a.n - 1]; };
+  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : n - 1]; };
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   struct Ids {
@@ -614,6 +624,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
     b2 = xadd(b2, __shfl_xor(b2, 16));
     b2 = xadd(b2, __shfl_xor(b2, 32));
     if (!qv[qb]) return -__builtin_inf();
+    if (TWO) return key_threshold(r0[ql[qb]], hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
     return mfma_threshold(hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
   };
   double thr[QB];
@@ -624,20 +635,24 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
   // bit (ids are non-negative).
   constexpr int QR = 4;
   unsigned pq_i[QR];
+  double pq_t[QR];  // two-stage: the pairs' MFMA keys
   int pq_n = 0;
 #pragma unroll
-  for (int r = 0; r < QR; ++r) pq_i[r] = 0u;
+  for (int r = 0; r < QR; ++r) {
+    pq_i[r] = 0u;
+    pq_t[r] = 0.0;
+  }
   dbl4 acc[QB][CB];
   // Apply the queues: every lane forms the oracle distances of its queued
   // pairs at once (rows read from global memory), then the four lane groups,
   // which cover the wave's queries once each, update their heaps in turn.
   auto flush = [&]() __attribute__((always_inline)) {
     double ed[QR];
-    const int qr0 = qrow(0), qr1 = qrow(QB - 1);
+    const int qr0 = TWO ? 0 : qrow(0), qr1 = TWO ? 0 : qrow(QB - 1);
 #pragma unroll
     for (int r = 0; r < QR; ++r) {
-      ed[r] = 0.0;
-      if (r < pq_n) {
+      ed[r] = pq_t[r];
+      if (!TWO && r < pq_n) {
         const bool hi_qb = pq_i[r] >> 31;
         const int id = (int)(pq_i[r] & 0x7fffffffu);
         const double *xq = a.XB + (size_t)(hi_qb ? qr1 : qr0) * d;
@@ -652,7 +667,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
           if (r < pq_n) {
             const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
             const int id = (int)(pq_i[r] & 0x7fffffffu);
-            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, ed[r], id);
+            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, kh, ed[r], id);
           }
         }
       }
@@ -751,12 +766,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
             const int b = __builtin_ctz(pass[qb]);
             pass[qb] &= pass[qb] - 1;
             int idx = cid[0];
+            double tv = acc[qb][0][0];
 #pragma unroll
             for (int bb = 1; bb < CB * 4; ++bb)
-              if (b == bb) idx = cid[bb];
+              if (b == bb) {
+                idx = cid[bb];
+                if (TWO) tv = acc[qb][bb >> 2][bb & 3];
+              }
 #pragma unroll
             for (int r = 0; r < QR; ++r)
-              if (pq_n == r) pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+              if (pq_n == r) {
+                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+                if (TWO) pq_t[r] = tv;
+              }
             ++pq_n;
           }
         }
@@ -785,7 +807,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 
   if (__any(pq_n > 0)) flush();
   __syncthreads();
-  if (tid < Q && qbase + tid < a.n) heap_store(a, hd, hi, Q, tid, qbase + tid, split);
+  if (tid < Q && qbase + tid < n) {
+    if (TWO)
+      key_heap_store(a, hd, hi, Q, tid, qbase + tid, kh);
+    else
+      heap_store(a, hd, hi, Q, tid, qbase + tid, split);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1163,6 +1190,91 @@ __global__ __launch_bounds__(256) void knn_merge_wave_kernel(KnnArgs<T> a) {
   }
 }
 
+// Two-stage FP64 MFMA form, stage 2 (the contract is stated with
+// key_heap_init): one wave per query.  Lanes form the oracle distances of
+// the key heap's entries (one entry per lane, rows read from global memory;
+// many waves per SIMD hide the latency), the wave sorts resident row + heap
+// entries (P = next power of two >= k + k2, padding (+inf, INT_MAX) last)
+// with a bitonic network in LDS and takes the first k.  The row is written
+// (descending, as every kernel of this library leaves it) when the heap
+// never filled or its root key T exceeds mfma_threshold(E) for the k-th
+// distance E; otherwise the row stays as it was and the query goes to the
+// fallback list (its XB row and heap row), for the one-stage kernel.
+__global__ __launch_bounds__(256) void knn_two_stage_finalize_kernel(KnnArgs<double> a, int terms,
+                                                                     int P, int *fail_n,
+                                                                     int *fail_b, int *fail_h) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  double *sd = (double *)smem + (size_t)w * P;
+  int *si = (int *)(smem + al16(sizeof(double) * 4 * (size_t)P)) + (size_t)w * P;
+  if (j >= a.n) return;  // whole wave; only wave barriers below
+  const int k = a.k, kh = a.k2, d = a.d;
+  const int hrow = a.hrow ? a.hrow[j] : j;
+  const size_t row = (size_t)hrow * a.ldk;
+  const int qr = a.bmap[j];
+  const double *xq = a.XB + (size_t)qr * d;
+  const double qn = a.XB2[qr];
+  const double *hk = a.hk + (size_t)j * kh;
+  const int *hid = a.hid + (size_t)j * kh;
+  for (int e = lane; e < kh; e += 64) {
+    const int id = hid[e];
+    double x = __builtin_inf();
+    if (id != INT_MAX) x = oracle_dist(a.XA + (size_t)id * d, xq, d, a.XA2[id], qn);
+    sd[k + e] = x;
+    si[k + e] = id;
+  }
+  for (int e = lane; e < k; e += 64) {
+    sd[e] = a.D[row + e];
+    si[e] = a.I[row + e];
+  }
+  for (int e = k + kh + lane; e < P; e += 64) {
+    sd[e] = __builtin_inf();
+    si[e] = INT_MAX;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int half = P / 2;
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = lane; q < half; q += 64) {
+        const int lo = 2 * stride * (q / stride) + q % stride;
+        const int hi = lo + stride;
+        const double x = sd[lo], y = sd[hi];
+        const int ix = si[lo], iy = si[hi];
+        const bool up = (lo & size) == 0;
+        if (up ? cless(y, iy, x, ix) : cless(x, ix, y, iy)) {
+          sd[lo] = y;
+          sd[hi] = x;
+          si[lo] = iy;
+          si[hi] = ix;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  bool ok = true;
+  if (hid[0] != INT_MAX) {
+    // The query's squared norm for the margin (any rounding order: the
+    // margin constant carries the slack).
+    double b2 = 0.0;
+    for (int p = lane; p < d; p += 64) b2 = __builtin_fma(xq[p], xq[p], b2);
+    for (int off = 32; off >= 1; off >>= 1) b2 = xadd(b2, __shfl_xor(b2, off));
+    const double cm = mfma_margin_c(terms, d);
+    ok = hk[0] > mfma_threshold(sd[k - 1], qn, xmul(xmul(0.5, cm), b2));
+  }
+  if (ok) {
+    for (int t = lane; t < k; t += 64) {
+      a.D[row + t] = sd[k - 1 - t];
+      a.I[row + t] = si[k - 1 - t];
+    }
+  } else if (lane == 0) {
+    const int f = atomicAdd(fail_n, 1);
+    fail_b[f] = qr;
+    fail_h[f] = hrow;
+  }
+}
+
 // Rows to ascending (distance, index) order, in place: each 64-lane
 // workgroup sorts R rows of P = next power of two >= k entries with a
 // bitonic network in LDS (padding (+inf, INT_MAX) sorts last).
@@ -1224,8 +1336,11 @@ __global__ void sqnorm_kernel(int N, int d, const T *X, T *X2) {
 // loads, a wave sum; any summation order, the margin has slack for it).
 __global__ __launch_bounds__(256) void cand_shift_kernel(int m, int d, int terms, const double *XA,
                                                          const double *XA2, const int *amap,
-                                                         double *SN) {
+                                                         double *SN, int *zero) {
   const int lane = threadIdx.x & 63;
+  // The two-stage form's fallback counter, cleared by a kernel (a launch
+  // sequence captured into a HIP graph replays it).
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   const int nw = gridDim.x * 4;
   const double cm = mfma_margin_c(terms, d);
   for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < m; i += nw) {
@@ -1307,6 +1422,11 @@ struct Plan {
   int small = 0;                      // exact, one wave per query: candidate slots per lane
   size_t lds = 0;
   int gx = 0, nsplit = 1, cps = 0;
+  // FP64 MFMA per-wave forms, unsplit: the two-stage form's heap size and
+  // LDS bytes (k2 = 0: not available).  The launch takes it when the
+  // workspace holds its heap lists.
+  int k2 = 0;
+  size_t lds2 = 0;
 };
 
 int exact_ct(int d, int elem) {
@@ -1412,7 +1532,7 @@ int mfma_blocks_per_cu(int ks, int cfg) {
 // an instantiation that spills.
 template <int KS, bool LINE>
 int mfma_direct_reg_blocks() {
-  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, direct_qb(KS), direct_cb(KS), 1, LINE>;
+  const void *kern = (const void *)knn_mfma_f64_direct_kernel<KS, direct_qb(KS), direct_cb(KS), 1, LINE, false>;
   hipFuncAttributes at;
   if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;
   int b = 0;
@@ -1498,7 +1618,7 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     }();
     if (direct_on && ring_env != 0 && (d > 64 || ring_env == 2) && pinned < 0 &&
         mode != MODE_MFMA_TILE && ring_supported(d)) {
-      const size_t b = ring_lds_bytes(d, k);
+      const size_t b = ring_lds_bytes(d, k, false);
       if (b <= kLdsMax) {
         const int blocks = std::min((int)(kLdsMax / b), ring_reg_blocks(d, d % 2 == 0));
         if (blocks > 0) {
@@ -1589,6 +1709,26 @@ Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
     p.nsplit = 1;
     p.cps = m;
   }
+  // The two-stage form (key heaps of k2 entries, oracle distances formed
+  // afterwards for those entries only) for unsplit operand-ring launches,
+  // when its heaps keep the same residency: measured 25 .. 30 % faster than
+  // one stage at d = 128 and 256 with empty rows, 7 .. 8 % with carried-over
+  // rows.  The direct form (d <= 64) stays in one stage: with the config-2
+  // carried-over rows the finalize pass costs more than it saves (9.71
+  // against 9.53 ms).  GSKNN_MFMA_TWO=0 turns the form off, =2 takes it for
+  // the direct form too (tuning).
+  static const int two_env = [] {
+    const char *e = getenv("GSKNN_MFMA_TWO");
+    return e ? atoi(e) : 1;
+  }();
+  if (two_env != 0 && p.kind == 1 && p.direct && (p.ring || two_env == 2) && p.nsplit == 1) {
+    const int k2 = two_stage_k2(k);
+    const size_t b = p.ring ? ring_lds_bytes(d, k, true) : MfmaHeapLds(p.qpb, k2, true).bytes;
+    if (b <= kLdsMax && (int)(kLdsMax / b) >= p.per_cu && two_stage_sort_size(k, k2) <= kTwoMaxP) {
+      p.k2 = k2;
+      p.lds2 = b;
+    }
+  }
   // Exact calls too small to fill the chip with one thread per query
   // (they would split the candidates and merge) and with m <= 2048, k <= 64,
   // d <= 64:
@@ -1614,19 +1754,21 @@ template <typename K>
 hipError_t set_lds(K kernel, size_t bytes);
 
 template <int KS>
-hipError_t launch_mfma_direct_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s) {
+hipError_t launch_mfma_direct_ks(const Plan &p, const KnnArgs<double> &a, hipStream_t s, bool two) {
   const dim3 grid(p.gx, p.nsplit);
   hipError_t e = hipSuccess;
   constexpr int QB = direct_qb(KS), CB = direct_cb(KS);
-#define GSKNN_DIRECT_CASE(W_, LINE_)                                          \
-  if (p.w == W_ && (a.d == 4 * KS) == LINE_) {                                \
-    auto kern = knn_mfma_f64_direct_kernel<KS, QB, CB, W_, LINE_>;            \
+#define GSKNN_DIRECT_CASE(W_, LINE_, TWO_)                                    \
+  if (p.w == W_ && (a.d == 4 * KS) == LINE_ && two == TWO_) {                 \
+    auto kern = knn_mfma_f64_direct_kernel<KS, QB, CB, W_, LINE_, TWO_>;      \
     if ((e = set_lds(kern, p.lds)) != hipSuccess) return e;                   \
     hipLaunchKernelGGL(kern, grid, dim3(64 * W_), p.lds, s, a);               \
     return hipGetLastError();                                                 \
   }
-  GSKNN_DIRECT_CASE(1, true)
-  GSKNN_DIRECT_CASE(1, false)
+  GSKNN_DIRECT_CASE(1, true, false)
+  GSKNN_DIRECT_CASE(1, false, false)
+  GSKNN_DIRECT_CASE(1, true, true)
+  GSKNN_DIRECT_CASE(1, false, true)
 #undef GSKNN_DIRECT_CASE
   return hipErrorNotSupported;
 }
@@ -1757,6 +1899,20 @@ hipError_t launch_merge(const KnnArgs<T> &a, hipStream_t s) {
   return hipErrorNotSupported;
 }
 
+// Two-stage workspace after the shifted norms: key heaps [n][k2] (keys, then
+// ids), the fallback count and the fallback lists (XB rows, heap rows).
+struct TwoLayout {
+  size_t hk, hid, cnt, fb, fh, bytes;
+  TwoLayout(size_t base, int n, int k2) {
+    hk = base;
+    hid = hk + al16(sizeof(double) * (size_t)n * k2);
+    cnt = hid + al16(sizeof(int) * (size_t)n * k2);
+    fb = cnt + 16;
+    fh = fb + al16(sizeof(int) * (size_t)n);
+    bytes = fh + al16(sizeof(int) * (size_t)n) - base;
+  }
+};
+
 size_t part_bytes(const Plan &p, int n, int k, int elem) {
   if (p.nsplit <= 1) return 0;
   const size_t ents = (size_t)p.nsplit * k * n;
@@ -1787,38 +1943,71 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
   const size_t a2b = a2_bytes(p, a.m);
   const size_t need = a2b + part_bytes(p, a.n, a.k, (int)sizeof(T));
   if (need > 0 && (work == nullptr || work_bytes < need)) return hipErrorInvalidValue;
+  const bool two = p.k2 > 0 && work_bytes >= a2b + TwoLayout(a2b, a.n, p.k2).bytes;
   if (p.nsplit > 1) {
     const size_t ents = (size_t)p.nsplit * a.k * a.n;
     a.partD = (T *)((char *)work + a2b);
     a.partI = (int *)((char *)work + a2b + al16(ents * sizeof(T)));
   }
   hipError_t e;
+  const int terms = p.ring ? ring_terms(a.d) : 4 * p.ks;
+  int *fail_n = nullptr;
+  if (two) {
+    const TwoLayout L(a2b, a.n, p.k2);
+    a.k2 = p.k2;
+    a.hk = (T *)((char *)work + L.hk);
+    a.hid = (int *)((char *)work + L.hid);
+    fail_n = (int *)((char *)work + L.cnt);
+  }
   if (a2b > 0) {
     if constexpr (sizeof(T) == 8) {
       a.SN = (const T *)work;
-      const int terms = p.ring ? ring_terms(a.d) : 4 * p.ks;
       const int blocks = std::min((a.m + 3) / 4, 2048);
       hipLaunchKernelGGL(cand_shift_kernel, dim3(blocks), dim3(256), 0, s, a.m, a.d, terms, a.XA,
-                         a.XA2, a.amap, (double *)work);
+                         a.XA2, a.amap, (double *)work, fail_n);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }
-  if (p.kind == 1 && p.ring) {
+  if (p.kind == 1 && p.direct) {
     if constexpr (sizeof(T) == 8) {
-      // 16-byte row loads need d even and both tables 16-byte aligned.
+      // 16-byte row loads (ring form) need d even and both tables 16-byte
+      // aligned.
       const bool vec = a.d % 2 == 0 && ((uintptr_t)a.XA % 16) == 0 && ((uintptr_t)a.XB % 16) == 0;
-      e = launch_knn_ring(a, vec, p.gx, p.lds, s);
-    } else {
-      return hipErrorNotSupported;
-    }
-  } else if (p.kind == 1 && p.direct) {
-    if constexpr (sizeof(T) == 8) {
-      switch (p.ks) {
-        case 4: e = launch_mfma_direct_ks<4>(p, a, s); break;
-        case 8: e = launch_mfma_direct_ks<8>(p, a, s); break;
-        case 16: e = launch_mfma_direct_ks<16>(p, a, s); break;
-        case 32: e = launch_mfma_direct_ks<32>(p, a, s); break;
-        default: return hipErrorNotSupported;
+      auto per_wave = [&](const KnnArgs<T> &x, bool two_stage) -> hipError_t {
+        if (p.ring)
+          return launch_knn_ring(x, vec, p.gx, two_stage ? p.lds2 : p.lds, s, two_stage);
+        Plan q = p;
+        q.lds = two_stage ? p.lds2 : p.lds;
+        switch (p.ks) {
+          case 4: return launch_mfma_direct_ks<4>(q, x, s, two_stage);
+          case 8: return launch_mfma_direct_ks<8>(q, x, s, two_stage);
+          case 16: return launch_mfma_direct_ks<16>(q, x, s, two_stage);
+          case 32: return launch_mfma_direct_ks<32>(q, x, s, two_stage);
+          default: return hipErrorNotSupported;
+        }
+      };
+      e = per_wave(a, two);
+      if (e == hipSuccess && two) {
+        const TwoLayout L(a2b, a.n, p.k2);
+        int *fail_b = (int *)((char *)work + L.fb);
+        int *fail_h = (int *)((char *)work + L.fh);
+        const int P = two_stage_sort_size(a.k, p.k2);
+        const size_t lds = al16(sizeof(double) * 4 * (size_t)P) + sizeof(int) * 4 * (size_t)P;
+        if ((e = set_lds(knn_two_stage_finalize_kernel, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(knn_two_stage_finalize_kernel, dim3((a.n + 3) / 4), dim3(256), lds, s, a,
+                           terms, P, fail_n, fail_b, fail_h);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // Queries whose rows the finalize kernel could not settle: the
+        // one-stage kernel over the fallback list (workgroups past the
+        // list's length return at once).
+        KnnArgs<T> f = a;
+        f.bmap = fail_b;
+        f.hrow = fail_h;
+        f.ncount = fail_n;
+        f.k2 = 0;
+        f.hk = nullptr;
+        f.hid = nullptr;
+        e = per_wave(f, false);
       }
     } else {
       return hipErrorNotSupported;
@@ -1847,6 +2036,22 @@ hipError_t launch_knn(KnnArgs<T> a, int mode, hipStream_t s, void *work, size_t 
 }  // namespace
 
 size_t knn_work_bytes(int m, int n, int d, int k, int elem_bytes, int mode) {
+  if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return 0;
+  if (elem_bytes == 4 && use_f32_mfma(m, d, k, mode)) return f32_work_bytes(m, n, d, k);
+  const Plan p = make_plan(m, n, d, k, elem_bytes, mode);
+  const size_t two = p.k2 > 0 ? TwoLayout(0, n, p.k2).bytes : 0;
+  return a2_bytes(p, m) + part_bytes(p, n, k, elem_bytes) + two;
+}
+
+size_t knn_two_stage_count_offset(int m, int n, int d, int k, int mode) {
+  if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return (size_t)-1;
+  const Plan p = make_plan(m, n, d, k, 8, mode);
+  if (p.k2 == 0) return (size_t)-1;
+  const size_t a2b = a2_bytes(p, m);
+  return TwoLayout(a2b, n, p.k2).cnt;
+}
+
+size_t knn_work_bytes_min(int m, int n, int d, int k, int elem_bytes, int mode) {
   if (m <= 0 || n <= 0 || d <= 0 || k <= 0) return 0;
   if (elem_bytes == 4 && use_f32_mfma(m, d, k, mode)) return f32_work_bytes(m, n, d, k);
   const Plan p = make_plan(m, n, d, k, elem_bytes, mode);

@@ -97,7 +97,7 @@ __device__ __forceinline__ void oracle_dist_batch(const double *const (&c)[R],
   for (int r = 0; r < R; ++r) out[r] = xadd(xadd(xmul(s[r], -2.0), cn[r]), qn[r]);
 }
 
-template <int KP, int QB, int CB, int WPE, bool VEC>
+template <int KP, int QB, int CB, int WPE, bool VEC, bool TWO>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void knn_mfma_f64_ring_kernel(
     KnnArgs<double> a) {
   static_assert(QB <= 2, "query block fits one bit");
@@ -107,17 +107,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   typedef int idv __attribute__((ext_vector_type(CB * 4)));
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = a.k, d = a.d;
-  double *hd = (double *)smem;
-  int *hi = (int *)(smem + al16(sizeof(double) * (size_t)k * Q));
+  const int n = a.ncount ? *a.ncount : a.n;
+  const int qbase = blockIdx.x * Q;
+  if (qbase >= n) return;  // fallback launches over a device-side list
+  const int kh = TWO ? a.k2 : k;  // heap entries per query
+  const MfmaHeapLds hl(Q, kh, TWO);
+  double *hd = (double *)(smem + hl.hd);
+  int *hi = (int *)(smem + hl.hi);
+  double *r0 = (double *)(smem + hl.r0);  // two-stage: roots of the resident rows
 
   const int lane = threadIdx.x, grp = lane >> 4, cl = lane & 15;
-  const int qbase = blockIdx.x * Q;
   const int split = blockIdx.y;
   const int cbeg = split * a.cps;
   const int cend = min(a.m, cbeg + a.cps);
   const bool sentinel = a.nsplit > 1;
 
-  if (lane < Q && qbase + lane < a.n) heap_load(a, hd, hi, Q, lane, qbase + lane, sentinel);
+  if (lane < Q && qbase + lane < n) {
+    if (TWO)
+      key_heap_init(a, hd, hi, r0, Q, lane, qbase + lane, kh);
+    else
+      heap_load(a, hd, hi, Q, lane, qbase + lane, sentinel);
+  }
 
   const int JR = QS ? ((d + 7) / 8 + R - 1) / R * R : KP;  // pair positions per tile
   const double cm = mfma_margin_c(8 * JR, d);
@@ -132,8 +142,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   for (int qb = 0; qb < QB; ++qb) {
     ql[qb] = qb * 16 + cl;
     const int j = qbase + ql[qb];
-    qv[qb] = j < a.n;
-    const int row = a.bmap[qv[qb] ? j : a.n - 1];
+    qv[qb] = j < n;
+    const int row = a.bmap[qv[qb] ? j : n - 1];
     const double *xq = a.XB + (size_t)row * d;
     qp[qb] = xq;
     qn[qb] = qv[qb] ? a.XB2[row] : 0.0;
@@ -152,7 +162,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
   }
   // XB row of query block qb.
-  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : a.n - 1]; };
+  auto qrow = [&](int qb) -> int { return a.bmap[qv[qb] ? qbase + ql[qb] : n - 1]; };
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
   struct Ids {
@@ -233,6 +243,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       b2 = xadd(b2, __shfl_xor(b2, 32));
     }
     if (!qv[qb]) return -__builtin_inf();
+    if (TWO) return key_threshold(r0[ql[qb]], hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
     return mfma_threshold(hd[ql[qb]], qn[qb], xmul(xmul(0.5, cm), b2));
   };
   double thr[QB];
@@ -242,26 +253,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   // Per-lane queue of passing pairs: candidate id, query block in the top
   // bit (ids are non-negative).
   unsigned pq_i[QR];
+  double pq_t[QR];  // two-stage: the pairs' MFMA keys
   int pq_n = 0;
 #pragma unroll
-  for (int r = 0; r < QR; ++r) pq_i[r] = 0u;
+  for (int r = 0; r < QR; ++r) {
+    pq_i[r] = 0u;
+    pq_t[r] = 0.0;
+  }
   rdbl4 acc[QB][CB];
   auto flush = [&]() __attribute__((always_inline)) {
-    const double *qp0 = a.XB + (size_t)qrow(0) * d;
-    const double *qp1 = a.XB + (size_t)qrow(QB - 1) * d;
-    const double *cp[QR], *qp[QR];
-    double cnr[QR], qnr[QR], ed[QR];
+    double ed[QR];
+    if constexpr (TWO) {
 #pragma unroll
-    for (int r = 0; r < QR; ++r) {
-      const bool on = r < pq_n;
-      const bool hq = pq_i[r] >> 31;
-      const int id = on ? (int)(pq_i[r] & 0x7fffffffu) : safe_id;
-      cp[r] = a.XA + (size_t)id * d;
-      qp[r] = hq ? qp1 : qp0;
-      cnr[r] = a.XA2[id];
-      qnr[r] = hq ? qn[QB - 1] : qn[0];
+      for (int r = 0; r < QR; ++r) ed[r] = pq_t[r];
+    } else {
+      const double *qp0 = a.XB + (size_t)qrow(0) * d;
+      const double *qp1 = a.XB + (size_t)qrow(QB - 1) * d;
+      const double *cp[QR], *qp[QR];
+      double cnr[QR], qnr[QR];
+#pragma unroll
+      for (int r = 0; r < QR; ++r) {
+        const bool on = r < pq_n;
+        const bool hq = pq_i[r] >> 31;
+        const int id = on ? (int)(pq_i[r] & 0x7fffffffu) : safe_id;
+        cp[r] = a.XA + (size_t)id * d;
+        qp[r] = hq ? qp1 : qp0;
+        cnr[r] = a.XA2[id];
+        qnr[r] = hq ? qn[QB - 1] : qn[0];
+      }
+      oracle_dist_batch<QR, VEC, (KP <= 24 ? 2 : 1)>(cp, qp, d, cnr, qnr, ed);
     }
-    oracle_dist_batch<QR, VEC, (KP <= 24 ? 2 : 1)>(cp, qp, d, cnr, qnr, ed);
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       if (grp == g) {
@@ -270,7 +291,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
           if (r < pq_n) {
             const int slot = (pq_i[r] >> 31) ? ql[QB - 1] : ql[0];
             const int id = (int)(pq_i[r] & 0x7fffffffu);
-            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, k, ed[r], id);
+            if (cless(ed[r], id, hd[slot], hi[slot])) sift_down(hd, hi, Q, slot, 0, kh, ed[r], id);
           }
         }
       }
@@ -399,12 +420,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const int b = __builtin_ctz(pass[qb]);
             pass[qb] &= pass[qb] - 1;
             int idx = cid[0];
+            double tv = acc[qb][0][0];
 #pragma unroll
             for (int bb = 1; bb < CB * 4; ++bb)
-              if (b == bb) idx = cid[bb];
+              if (b == bb) {
+                idx = cid[bb];
+                if (TWO) tv = acc[qb][bb >> 2][bb & 3];
+              }
 #pragma unroll
             for (int r = 0; r < QR; ++r)
-              if (pq_n == r) pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+              if (pq_n == r) {
+                pq_i[r] = (unsigned)idx | ((unsigned)qb << 31);
+                if (TWO) pq_t[r] = tv;
+              }
             ++pq_n;
           }
         }
@@ -434,7 +462,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
   if (__any(pq_n > 0)) flush();
   __syncthreads();
-  if (lane < Q && qbase + lane < a.n) heap_store(a, hd, hi, Q, lane, qbase + lane, split);
+  if (lane < Q && qbase + lane < n) {
+    if (TWO)
+      key_heap_store(a, hd, hi, Q, lane, qbase + lane, kh);
+    else
+      heap_store(a, hd, hi, Q, lane, qbase + lane, split);
+  }
 }
 
 struct RingCfg {
@@ -445,9 +478,9 @@ struct RingCfg {
 // one wave per SIMD (512 registers: the query operands of all pairs, the
 // ring, the accumulators and an exact-distance flush without spilling), two
 // query blocks per wave (each candidate operand feeds two MFMAs) and four
-// candidate blocks per tile up to d = 128 (the per-tile epilogue amortised
-// over 16 MFMAs per pair), two above, where the query operands take more
-// registers.
+// candidate blocks per tile up to d = 112 (the per-tile epilogue amortised
+// over 16 MFMAs per pair), two up to d = 192 and one up to d = 256, where
+// the query operands take more registers.
 constexpr int kRingCfg[][4] = {{2, 2, 4, 1},   {4, 2, 4, 1},   {6, 2, 4, 1},   {8, 2, 4, 1},
                                {10, 2, 4, 1},  {12, 2, 4, 1},  {14, 2, 4, 1},  {16, 2, 2, 1},
                                {20, 2, 2, 1},  {24, 2, 2, 1},  {28, 2, 1, 1},  {32, 2, 1, 1}};
@@ -469,13 +502,20 @@ bool ring_cfg(int d, RingCfg *c) {
 
 template <int KP, int QB, int CB, int WPE, bool VEC>
 int ring_reg_blocks_t() {
-  const void *kern = (const void *)knn_mfma_f64_ring_kernel<KP, QB, CB, WPE, VEC>;
-  hipFuncAttributes at;
-  if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;  // spills
-  int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, 64, 0) != hipSuccess || b <= 0)
-    b = 4 * WPE;
-  return b;
+  // Both forms must run without spilling; the one-stage form re-runs the
+  // two-stage form's unsettled queries.
+  const void *kerns[2] = {(const void *)knn_mfma_f64_ring_kernel<KP, QB, CB, WPE, VEC, false>,
+                          (const void *)knn_mfma_f64_ring_kernel<KP, QB, CB, WPE, VEC, true>};
+  int best = 1 << 30;
+  for (const void *kern : kerns) {
+    hipFuncAttributes at;
+    if (hipFuncGetAttributes(&at, kern) == hipSuccess && at.localSizeBytes > 0) return 0;  // spills
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, 64, 0) != hipSuccess || b <= 0)
+      b = 4 * WPE;
+    best = std::min(best, b);
+  }
+  return best;
 }
 
 #define GSKNN_RING_SHAPES(X) \
@@ -516,9 +556,9 @@ int ring_candidates_per_tile(int d) {
   return ring_cfg(d, &c) ? 16 * c.cb : 0;
 }
 
-size_t ring_lds_bytes(int d, int k) {
+size_t ring_lds_bytes(int d, int k, bool two) {
   const int Q = ring_queries_per_block(d);
-  return al16(sizeof(double) * (size_t)k * Q) + sizeof(int) * (size_t)k * Q;
+  return MfmaHeapLds(Q, two ? two_stage_k2(k) : k, two).bytes;
 }
 
 // Workgroups per CU that registers allow (0: the instantiation spills),
@@ -541,14 +581,17 @@ int ring_reg_blocks(int d, bool vec) {
   return b;
 }
 
-hipError_t launch_knn_ring(const KnnArgs<double> &a, bool vec, int gx, size_t lds, hipStream_t s) {
+hipError_t launch_knn_ring(const KnnArgs<double> &a, bool vec, int gx, size_t lds, hipStream_t s,
+                           bool two) {
   RingCfg c;
   if (!ring_cfg(a.d, &c)) return hipErrorNotSupported;
   const dim3 grid(gx, a.nsplit);
 #define GSKNN_RING_LAUNCH(KP_, QB_, CB_, WPE_)                                          \
   if (c.kp == KP_) {                                                                    \
-    auto kern = vec ? knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, true>               \
-                    : knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, false>;             \
+    auto kern = vec ? (two ? knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, true, true>   \
+                           : knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, true, false>) \
+                    : (two ? knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, false, true>  \
+                           : knn_mfma_f64_ring_kernel<KP_, QB_, CB_, WPE_, false, false>); \
     hipError_t e = hipFuncSetAttribute((const void *)kern,                              \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     if (e != hipSuccess) return e;                                                      \

@@ -79,6 +79,8 @@ def _declare(L):
     L.gsknn_dknn_work_bytes.restype = sz
     L.gsknn_sknn_overflow_offset.argtypes = [i32] * 5
     L.gsknn_sknn_overflow_offset.restype = sz
+    L.gsknn_dknn_fallback_offset.argtypes = [i32] * 5
+    L.gsknn_dknn_fallback_offset.restype = sz
     L.gsknn_sknn_work_bytes.argtypes = [i32] * 5
     L.gsknn_sknn_work_bytes.restype = sz
     L.gsknn_bf16_prep_bytes.argtypes = [i32, i32]

@@ -108,9 +108,16 @@ def test_malloc_aligned():
 def test_work_bytes_planning_is_host_only():
     # Split planning: few queries, many candidates -> partial-list workspace.
     assert gsknn.work_bytes_d(2_000_000, 10, 128, 32) > 2_000_000 * 8
-    # An unsplit FP64 MFMA launch needs only the candidates' shifted norms
-    # (one double per candidate position).
-    assert gsknn.work_bytes_d(65536, 65536, 64, 32) == 65536 * 8
+    # An unsplit FP64 MFMA launch of the direct form (d <= 64, one stage)
+    # needs only the candidates' shifted norms (one double per candidate
+    # position).
+    n, k2 = 65536, 40
+    assert gsknn.work_bytes_d(65536, n, 64, 32) == 65536 * 8
+    # The operand-ring form (d > 64) runs in two stages: the shifted norms,
+    # then key heaps of k2 = 40 entries per query (8-byte key, 4-byte id),
+    # the fallback count (16 bytes) and the two fallback lists (4 bytes per
+    # query each).
+    assert gsknn.work_bytes_d(65536, n, 96, 32) == 65536 * 8 + n * k2 * 12 + 16 + 2 * n * 4
     assert gsknn.work_bytes_d(0, 10, 8, 4) == 0
 
 

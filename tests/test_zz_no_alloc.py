@@ -1,8 +1,10 @@
 """gsknn_dknn_noalloc_dev (the entry point that allocates nothing) through
 the C-ABI: its rows equal gsknn_dknn_dev's bit for bit on two calls in a
-row that use the fixed scratch array (candidate-split launches), the second
-smaller than the first; a call needing more scratch than the array holds
-is refused with GSKNN_ELIMIT before any work is enqueued."""
+row that use the fixed scratch array (candidate-split launches, and the
+two-stage form's key heaps), the second smaller than the first; a call
+whose key heaps exceed the array runs in one stage; a call needing more
+scratch than the array holds even then is refused with GSKNN_ELIMIT before
+any work is enqueued."""
 import numpy as np
 import pytest
 
@@ -46,6 +48,30 @@ def test_noalloc_matches_dknn_dev_two_calls(mode):
         rc, Dn, In = _rows(t, k, n, gsknn.dknn_noalloc_dev, mode=mode)
         assert rc == 0, gsknn.last_error()
         _, Dr, Ir = _rows(t, k, n, lambda *a, **kw: gsknn.dknn_dev(*a, **kw), mode=mode)
+        np.testing.assert_array_equal(In, Ir)
+        assert np.array_equal(Dn.view(np.uint8), Dr.view(np.uint8))
+
+
+def test_noalloc_two_stage_form():
+    # Unsplit operand-ring launches (d > 64, enough queries to fill the chip)
+    # take the two-stage form, whose key heaps and fallback lists live in the
+    # scratch array too: two calls in a row, the second smaller.  The third
+    # call's key heaps (200000 queries x 40 entries x 12 bytes) exceed the
+    # array, so it runs in one stage there; its rows still equal those of
+    # gsknn_dknn_dev, which takes the two-stage form.
+    g = np.random.Generator(np.random.PCG64(11))
+    L = gsknn.lib()
+    for (N, d, m, n, k) in [(60000, 96, 2048, 32768, 32), (50000, 128, 1024, 32768, 24),
+                            (250000, 96, 1024, 200000, 32)]:
+        two = L.gsknn_dknn_fallback_offset(m, n, d, k, gsknn.MODE_AUTO) != (1 << 64) - 1
+        need = L.gsknn_dknn_work_bytes(m, n, d, k, gsknn.MODE_AUTO)
+        print(f"[gpu] noalloc two-stage m={m} n={n} d={d} k={k} two={two} need={need}", flush=True)
+        assert two
+        assert (need <= gsknn.NOALLOC_SCRATCH_BYTES) == (n < 100000)
+        t = _case(g, N, d, m, n, k)
+        rc, Dn, In = _rows(t, k, n, gsknn.dknn_noalloc_dev)
+        assert rc == 0, gsknn.last_error()
+        _, Dr, Ir = _rows(t, k, n, lambda *a, **kw: gsknn.dknn_dev(*a, **kw))
         np.testing.assert_array_equal(In, Ir)
         assert np.array_equal(Dn.view(np.uint8), Dr.view(np.uint8))
 
