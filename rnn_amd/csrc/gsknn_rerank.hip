@@ -30,7 +30,8 @@
 // final k-th distance from above.  Every candidate whose lower bound is <= U
 // at the time it is seen is appended to the query's list (position, lower).
 // Pass 2 (knn_rerank_kernel): one wave per query reads its list 64 entries
-// at a time, drops entries with lower > final U, computes sum_p (a_p - b_p)^2
+// at a time, drops entries with lower > final U or above the row's current
+// k-th distance (which falls as entries are merged), computes sum_p (a_p - b_p)^2
 // in FP64 for the rest (four at a time, 16 lanes each, fixed summation order)
 // and merges them into the resident row under the canonical order.  A query
 // whose list overflowed is re-scanned against every candidate in FP64.
@@ -155,14 +156,14 @@ struct FilterLayout {
     xn = al16r(nc + C * 4);
     en = al16r(xn + C * 4);
     heap = al16r(en + C * 4);
-    cnt = al16r(heap + (size_t)k * Q * 4);
+    cnt = al16r(heap + (size_t)4 * Q * ((k + 3) / 4) * 4);
     bytes = al16r(cnt + (size_t)Q * 4);
   }
 };
 
 
-__device__ void sift_vals(float *h, int QS, int q, int k, float x) {
-  int s = 0;
+// Sift x down from slot s of the max-heap of k floats at h[q], h[q + QS], ...
+__device__ void sift_vals(float *h, int QS, int q, int k, float x, int s = 0) {
   while (true) {
     int c = 2 * s + 1;
     if (c >= k) break;
@@ -187,11 +188,12 @@ __device__ void sift_vals(float *h, int QS, int q, int k, float x) {
 //
 // Hot path per pair: tt = fma(-2, dot~, nc) and a running minimum; a query
 // block whose minimum reaches the threshold thr = U - nq + delta_max
-// (delta_max bounds delta over all candidates) compares its pairs.  Survivors get their exact lower/upper bounds and wait in a
-// per-lane queue of QR entries; the queue is applied (list append, upper-bound
-// heap update) only when some lane's queue would overflow.  A stale U only
-// admits more candidates; every queued candidate is tested against the live U
-// when applied.
+// (delta_max bounds delta over all candidates) compares its pairs.
+// Survivors get their exact lower/upper bounds and wait in a per-lane queue
+// of QR entries; the queue is applied (list append through an LDS counter,
+// update of the lane's own upper-bound heap) when some lane's queue would
+// overflow, by every lane at once, and U is then refreshed.  A stale U only
+// admits more candidates.
 // Two waves per SIMD when the query fragments (4 * QB * KS registers) leave
 // room for the rest.
 constexpr int bf16_waves_per_eu(int KS, int QB) { return QB * KS <= 16 ? 2 : 1; }
@@ -231,29 +233,28 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   const int cend = min(a.m, cbeg + a.cps);
   const size_t sbase = (size_t)split * a.n;  // this split's rows of lst/cnt/U
 
-  // Upper-bound heaps start from the resident rows (rounded up to FP32).
-  if (tid < Q) {
-    const int j = qbase + tid;
+  // Upper-bound heaps.  Lane group g (lanes 16 g .. 16 g + 15) keeps, for
+  // each of its queries, a max-heap of its own of kg = (k + 3 - g) / 4
+  // upper bounds, seeded with the resident row's entries g, g + 4, ...
+  // (rounded up to FP32) and fed with the upper bounds of the candidates
+  // that group's lanes see.  The four heaps of a query hold k values in all,
+  // each at or below the largest of the four roots, so that maximum U bounds
+  // the k-th distance from above.  Each lane updates only its own heaps, so
+  // every lane applies its survivors at once.
+  constexpr int NS = 4 * Q;  // heap slots: (lane, query block) of the workgroup
+  const int kg = (k + 3 - grp) / 4;
+  auto hslot = [&](int qb) -> int { return (w * 64 + lane) * QB + qb; };
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int j = qbase + w * QW + qb * 16 + (lane & 15);
+    const int hs = hslot(qb);
     if (j < a.n) {
       const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
-      for (int t = 0; t < k; ++t) hv[t * Q + tid] = __double2float_ru(a.D[row + t]);
-      for (int s = k / 2 - 1; s >= 0; --s) {
-        float x = hv[s * Q + tid];
-        int p = s;
-        while (true) {
-          int c = 2 * p + 1;
-          if (c >= k) break;
-          float cv = hv[c * Q + tid];
-          if (c + 1 < k && cv < hv[(c + 1) * Q + tid]) cv = hv[(++c) * Q + tid];
-          if (!(x < cv)) break;
-          hv[p * Q + tid] = cv;
-          p = c;
-        }
-        hv[p * Q + tid] = x;
-      }
+      for (int e = 0; e < kg; ++e) hv[e * NS + hs] = __double2float_ru(a.D[row + grp + 4 * e]);
+      for (int e = kg / 2 - 1; e >= 0; --e) sift_vals(hv, NS, hs, kg, hv[e * NS + hs], e);
     }
-    hc[tid] = 0;
   }
+  if (tid < Q) hc[tid] = 0;
 
   const float u = kU32;
   const float g = (float)((a.d + 1) * 2.0 * 0x1p-24 / (1.0 - (a.d + 1) * 2.0 * 0x1p-24));
@@ -371,10 +372,26 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     store_x(0);
   }
   __syncthreads();
-  float thr[QB];
+  // U of each query of this lane (every lane of the wave calls it).
+  auto group_u = [&](int qb) -> float {
+    float r = kg > 0 ? hv[hslot(qb)] : -__builtin_inff();
+    r = fmaxf(r, __shfl_xor(r, 16));
+    r = fmaxf(r, __shfl_xor(r, 32));
+    return r;
+  };
+  float uq[QB], thr[QB];
 #pragma unroll
-  for (int qb = 0; qb < QB; ++qb)
-    thr[qb] = qv[qb] ? hv[ql[qb]] - nq[qb] + dmx[qb] : -__builtin_inff();
+  for (int qb = 0; qb < QB; ++qb) {
+    uq[qb] = group_u(qb);
+    thr[qb] = qv[qb] ? uq[qb] - nq[qb] + dmx[qb] : -__builtin_inff();
+  }
+  auto refresh = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      uq[qb] = group_u(qb);
+      if (qv[qb]) thr[qb] = uq[qb] - nq[qb] + dmx[qb];
+    }
+  };
 
   // Per-lane queue: lower, upper, candidate position, query block.
   float pq_lo[QR], pq_up[QR];
@@ -385,38 +402,28 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     pq_lo[r] = pq_up[r] = 0.f;
     pq_ps[r] = pq_q[r] = 0;
   }
-  // Apply one candidate to query slot `slot`: append if it can still reach
-  // the k-th distance, tighten U with its upper bound.
-  auto apply = [&](int slot, float lower, float upper, int ps) {
-    const float Uq = hv[slot];
-    if (lower <= Uq) {
-      const int at = hc[slot]++;
+  // Apply one candidate of query block qb: append it if it can still reach
+  // the k-th distance (against U as of the last refresh: a stale U only
+  // admits more), tighten this lane's heap with its upper bound.
+  auto apply = [&](int qb, float lower, float upper, int ps) __attribute__((always_inline)) {
+    if (lower <= uq[qb]) {
+      const int at = atomicAdd(&hc[ql[qb]], 1);
       if (at < a.cap)
-        a.lst[(sbase + qbase + slot) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+        a.lst[(sbase + qbase + ql[qb]) * a.cap + at] = make_int2(ps, __float_as_int(lower));
     }
-    if (upper < Uq) sift_vals(hv, Q, slot, k, upper);
+    const int hs = hslot(qb);
+    if (kg > 0 && upper < hv[hs]) sift_vals(hv, NS, hs, kg, upper);
   };
   auto flush = [&]() {
-#pragma unroll 1
-    for (int gsel = 0; gsel < 4; ++gsel) {
-      if (grp == gsel) {
-        for (int r = 0; r < QR; ++r) {
-          if (r < pq_n) {
-            int slot = ql[0];
+    for (int r = 0; r < QR; ++r) {
+      if (r < pq_n) {
 #pragma unroll
-            for (int qb = 1; qb < QB; ++qb)
-              if (pq_q[r] == qb) slot = ql[qb];
-            apply(slot, pq_lo[r], pq_up[r], pq_ps[r]);
-          }
-        }
+        for (int qb = 0; qb < QB; ++qb)
+          if (pq_q[r] == qb) apply(qb, pq_lo[r], pq_up[r], pq_ps[r]);
       }
-      __builtin_amdgcn_wave_barrier();
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
     pq_n = 0;
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb)
-      if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+    refresh();
   };
 
   for (int t = 0; t < ntiles; ++t) {
@@ -476,34 +483,25 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
       // all of its survivors, every lane applies this tile's survivors now.
       const bool direct = pq_n + npass > QR;
       if (__any(direct)) {
-#pragma unroll 1
-        for (int gsel = 0; gsel < 4; ++gsel) {
-          if (grp == gsel) {
 #pragma unroll
-            for (int qb = 0; qb < QB; ++qb) {
-              unsigned bits = pass[qb];
-              while (bits) {
-                const int b = __builtin_ctz(bits);
-                bits &= bits - 1;
-                float tt = acc[qb][0][0];
+        for (int qb = 0; qb < QB; ++qb) {
+          unsigned bits = pass[qb];
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            float tt = acc[qb][0][0];
 #pragma unroll
-                for (int bb = 1; bb < CB * 4; ++bb)
-                  if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
-                const int c = (b >> 2) * 16 + out_row(lane, b & 3);
-                if (cbeg + t * C + c >= cend) continue;
-                const float dist = tt + nq[qb];
-                const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
-                                        1.0009765625f + 1e-30f;
-                apply(ql[qb], dist - delta, dist + delta, cbeg + t * C + c);
-              }
-            }
+            for (int bb = 1; bb < CB * 4; ++bb)
+              if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
+            const int c = (b >> 2) * 16 + out_row(lane, b & 3);
+            if (cbeg + t * C + c >= cend) continue;
+            const float dist = tt + nq[qb];
+            const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
+                                    1.0009765625f + 1e-30f;
+            apply(qb, dist - delta, dist + delta, cbeg + t * C + c);
           }
-          __builtin_amdgcn_wave_barrier();
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
         }
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb)
-          if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+        refresh();
       } else {
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
@@ -538,14 +536,20 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     __syncthreads();  // next tile visible
   }
   if (__any(pq_n > 0)) flush();
-  __syncthreads();
+  float uf[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) uf[qb] = group_u(qb);
+  __syncthreads();  // every append counted
 
-  if (tid < Q) {
-    const int j = qbase + tid;
-    if (j < a.n) {
-      a.U[sbase + j] = hv[tid];
-      a.cnt[sbase + j] = hc[tid];
-      if (hc[tid] > a.cap) atomicAdd(a.nover, 1);
+  if (grp == 0) {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const int j = qbase + ql[qb];
+      if (j < a.n) {
+        a.U[sbase + j] = uf[qb];
+        a.cnt[sbase + j] = hc[ql[qb]];
+        if (hc[ql[qb]] > a.cap) atomicAdd(a.nover, 1);
+      }
     }
   }
 }
@@ -627,6 +631,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
       // One list entry (or, for an overflowed list, one candidate) per lane.
       const int e = base + lane;
       int pos = 0;
+      float lw = -__builtin_inff();  // the entry's lower bound (none when re-scanning)
       bool keep = e < nscan;
       if (keep) {
         if (over) {
@@ -634,7 +639,8 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
         } else {
           const int2 en = a.lst[sj * a.cap + e];
           pos = en.x;
-          keep = __int_as_float(en.y) <= Ulim;
+          lw = __int_as_float(en.y);
+          keep = lw <= Ulim && (double)lw <= rd;
         }
       }
       unsigned long long live = __ballot(keep);
@@ -683,6 +689,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
             ri = hi[0];
           }
         }
+        // Entries whose lower bound exceeds the row's k-th distance can no
+        // longer enter it.
+        live &= __ballot((double)lw <= rd);
       }
     }
   }
