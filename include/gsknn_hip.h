@@ -25,16 +25,22 @@
 extern "C" {
 #endif
 
-/* Contraction selection.
+/* Contraction selection.  Every mode returns the reference oracle's
+ * distance bits and the canonical (distance, lower index) neighbour set.
  * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
- * reference oracle's arithmetic (bit-exact distances); MFMA = FP64 MFMA,
- * for d <= 128 the form that streams candidates into registers per wave
- * whenever the heaps fit its LDS (AUTO takes it only when it keeps at least
- * as many queries resident as the LDS-tile kernel); MFMA_TILE = FP64 MFMA
- * with the LDS-tile kernel only (same results within the FP64 tolerance).
+ * oracle's arithmetic; MFMA = FP64 MFMA wherever a per-wave form fits: for
+ * d <= 64 the form that streams candidates into registers per wave, above
+ * it the operand-ring form (query operands in registers up to d = 256,
+ * streamed beyond), in two stages when the launch is not split (key heaps,
+ * then oracle distances of their entries); AUTO takes the direct form only
+ * when it keeps at least as many queries resident as the LDS-tile kernel;
+ * MFMA_TILE = FP64 MFMA with the LDS-tile kernel only (d <= 256; exact
+ * VALU above).  The MFMA value only filters: every pair that can reach a
+ * row's k-th distance under a rigorous error margin has its distance formed
+ * in the oracle's arithmetic.
  * FP32: AUTO/MFMA = FP32 MFMA candidate pass + re-rank in the oracle's FP32
  * arithmetic when d >= 8 and m >= 64, else the exact VALU kernel; EXACT =
- * always the VALU kernel.  Every FP32 mode returns the oracle's distance bits. */
+ * always the VALU kernel. */
 #define GSKNN_MODE_AUTO 0
 #define GSKNN_MODE_EXACT 1
 #define GSKNN_MODE_MFMA 2

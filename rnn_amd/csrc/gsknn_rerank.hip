@@ -190,15 +190,14 @@ __device__ void sift_vals(float *h, int QS, int q, int k, float x, int s = 0) {
 // block whose minimum reaches the threshold thr = U - nq + delta_max
 // (delta_max bounds delta over all candidates) compares its pairs.
 // Survivors get their exact lower/upper bounds and wait in a per-lane queue
-// of QR entries; the queue is applied (list append through an LDS counter,
-// update of the lane's own upper-bound heap) when some lane's queue would
-// overflow, by every lane at once, and U is then refreshed.  A stale U only
-// admits more candidates.
+// of QR entries; the queue is applied (list append, upper-bound heap update)
+// when some lane's queue would overflow, and U is then refreshed.  A stale U
+// only admits more candidates.
 // Two waves per SIMD when the query fragments (4 * QB * KS registers) leave
 // room for the rest.
 constexpr int bf16_waves_per_eu(int KS, int QB) { return QB * KS <= 16 ? 2 : 1; }
 
-template <int KS, int QB, int CB, int W>
+template <int KS, int QB, int CB, int W, bool PL>
 __global__ __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_filter_kernel(Bf16Args a) {
   constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
@@ -233,26 +232,36 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   const int cend = min(a.m, cbeg + a.cps);
   const size_t sbase = (size_t)split * a.n;  // this split's rows of lst/cnt/U
 
-  // Upper-bound heaps.  Lane group g (lanes 16 g .. 16 g + 15) keeps, for
-  // each of its queries, a max-heap of its own of kg = (k + 3 - g) / 4
-  // upper bounds, seeded with the resident row's entries g, g + 4, ...
-  // (rounded up to FP32) and fed with the upper bounds of the candidates
-  // that group's lanes see.  The four heaps of a query hold k values in all,
-  // each at or below the largest of the four roots, so that maximum U bounds
-  // the k-th distance from above.  Each lane updates only its own heaps, so
-  // every lane applies its survivors at once.
-  constexpr int NS = 4 * Q;  // heap slots: (lane, query block) of the workgroup
-  const int kg = (k + 3 - grp) / 4;
+  // Upper-bound heaps (U, the k-th smallest upper bound seen, bounds the
+  // k-th distance from above), seeded with the resident row's distances
+  // rounded up to FP32.
+  //  * PL (per lane): lane group g (lanes 16 g .. 16 g + 15) keeps, for each
+  //    of its queries, a max-heap of its own of kg = (k + 3 - g) / 4 upper
+  //    bounds: the resident row's entries g, g + 4, ... and the candidates
+  //    that group's lanes see.  The four heaps of a query hold k values in
+  //    all, each at or below the largest of the four roots, so that maximum
+  //    is a valid U (somewhat above the exact one).  Each lane updates only
+  //    its own heaps, so every lane applies its survivors at once.
+  //  * otherwise one heap of k per query, the exact U; the four lane groups
+  //    holding a query's candidates take turns to update it.
+  constexpr int NS = 4 * Q;  // per-lane heap slots: (lane, query block) of the workgroup
+  const int kg = PL ? (k + 3 - grp) / 4 : k;
   auto hslot = [&](int qb) -> int { return (w * 64 + lane) * QB + qb; };
+  if constexpr (PL) {
 #pragma unroll
-  for (int qb = 0; qb < QB; ++qb) {
-    const int j = qbase + w * QW + qb * 16 + (lane & 15);
-    const int hs = hslot(qb);
-    if (j < a.n) {
-      const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
-      for (int e = 0; e < kg; ++e) hv[e * NS + hs] = __double2float_ru(a.D[row + grp + 4 * e]);
-      for (int e = kg / 2 - 1; e >= 0; --e) sift_vals(hv, NS, hs, kg, hv[e * NS + hs], e);
+    for (int qb = 0; qb < QB; ++qb) {
+      const int j = qbase + w * QW + qb * 16 + (lane & 15);
+      const int hs = hslot(qb);
+      if (j < a.n) {
+        const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+        for (int e = 0; e < kg; ++e) hv[e * NS + hs] = __double2float_ru(a.D[row + grp + 4 * e]);
+        for (int e = kg / 2 - 1; e >= 0; --e) sift_vals(hv, NS, hs, kg, hv[e * NS + hs], e);
+      }
     }
+  } else if (tid < Q && qbase + tid < a.n) {
+    const size_t row = (size_t)(a.hrow ? a.hrow[qbase + tid] : qbase + tid) * a.ldk;
+    for (int e = 0; e < k; ++e) hv[e * Q + tid] = __double2float_ru(a.D[row + e]);
+    for (int e = k / 2 - 1; e >= 0; --e) sift_vals(hv, Q, tid, k, hv[e * Q + tid], e);
   }
   if (tid < Q) hc[tid] = 0;
 
@@ -374,6 +383,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   __syncthreads();
   // U of each query of this lane (every lane of the wave calls it).
   auto group_u = [&](int qb) -> float {
+    if constexpr (!PL) return hv[ql[qb]];
     float r = kg > 0 ? hv[hslot(qb)] : -__builtin_inff();
     r = fmaxf(r, __shfl_xor(r, 16));
     r = fmaxf(r, __shfl_xor(r, 32));
@@ -403,25 +413,47 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     pq_ps[r] = pq_q[r] = 0;
   }
   // Apply one candidate of query block qb: append it if it can still reach
-  // the k-th distance (against U as of the last refresh: a stale U only
-  // admits more), tighten this lane's heap with its upper bound.
+  // the k-th distance, tighten the heap with its upper bound.  PL: against U
+  // as of the last refresh (a stale U only admits more), appended through
+  // an LDS counter, this lane's heap; otherwise against the live root of the
+  // query's heap (the caller's lane group holds it alone).
   auto apply = [&](int qb, float lower, float upper, int ps) __attribute__((always_inline)) {
-    if (lower <= uq[qb]) {
-      const int at = atomicAdd(&hc[ql[qb]], 1);
+    const float Uq = PL ? uq[qb] : hv[ql[qb]];
+    if (lower <= Uq) {
+      const int at = PL ? atomicAdd(&hc[ql[qb]], 1) : hc[ql[qb]]++;
       if (at < a.cap)
         a.lst[(sbase + qbase + ql[qb]) * a.cap + at] = make_int2(ps, __float_as_int(lower));
     }
-    const int hs = hslot(qb);
-    if (kg > 0 && upper < hv[hs]) sift_vals(hv, NS, hs, kg, upper);
+    if constexpr (PL) {
+      const int hs = hslot(qb);
+      if (kg > 0 && upper < hv[hs]) sift_vals(hv, NS, hs, kg, upper);
+    } else {
+      if (upper < Uq) sift_vals(hv, Q, ql[qb], k, upper);
+    }
   };
-  auto flush = [&]() {
-    for (int r = 0; r < QR; ++r) {
-      if (r < pq_n) {
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb)
-          if (pq_q[r] == qb) apply(qb, pq_lo[r], pq_up[r], pq_ps[r]);
+  // Lane groups one after another (shared heaps), or all at once (PL).
+  auto by_groups = [&](auto &&body) __attribute__((always_inline)) {
+    if constexpr (PL) {
+      body();
+    } else {
+#pragma unroll 1
+      for (int gsel = 0; gsel < 4; ++gsel) {
+        if (grp == gsel) body();
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
     }
+  };
+  auto flush = [&]() {
+    by_groups([&]() {
+      for (int r = 0; r < QR; ++r) {
+        if (r < pq_n) {
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb)
+            if (pq_q[r] == qb) apply(qb, pq_lo[r], pq_up[r], pq_ps[r]);
+        }
+      }
+    });
     pq_n = 0;
     refresh();
   };
@@ -483,24 +515,26 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
       // all of its survivors, every lane applies this tile's survivors now.
       const bool direct = pq_n + npass > QR;
       if (__any(direct)) {
+        by_groups([&]() {
 #pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          unsigned bits = pass[qb];
-          while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            float tt = acc[qb][0][0];
+          for (int qb = 0; qb < QB; ++qb) {
+            unsigned bits = pass[qb];
+            while (bits) {
+              const int b = __builtin_ctz(bits);
+              bits &= bits - 1;
+              float tt = acc[qb][0][0];
 #pragma unroll
-            for (int bb = 1; bb < CB * 4; ++bb)
-              if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
-            const int c = (b >> 2) * 16 + out_row(lane, b & 3);
-            if (cbeg + t * C + c >= cend) continue;
-            const float dist = tt + nq[qb];
-            const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
-                                    1.0009765625f + 1e-30f;
-            apply(qb, dist - delta, dist + delta, cbeg + t * C + c);
+              for (int bb = 1; bb < CB * 4; ++bb)
+                if (b == bb) tt = acc[qb][bb >> 2][bb & 3];
+              const int c = (b >> 2) * 16 + out_row(lane, b & 3);
+              if (cbeg + t * C + c >= cend) continue;
+              const float dist = tt + nq[qb];
+              const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
+                                      1.0009765625f + 1e-30f;
+              apply(qb, dist - delta, dist + delta, cbeg + t * C + c);
+            }
           }
-        }
+        });
         refresh();
       } else {
 #pragma unroll
@@ -717,6 +751,7 @@ constexpr int kCBb = 4;
 
 struct Bf16Plan {
   int ks = 0, qb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
+  bool pl = false;  // per-lane upper-bound heaps
   size_t lds = 0;
 };
 
@@ -769,8 +804,15 @@ Bf16Plan plan_bf16(int d, int k) {
       break;
     }
   }
-  int cap = 16 * k;
-  p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
+  // Per-lane heaps from k = 48 on (12 entries or more per lane): at config 3
+  // (k = 64) the filter runs 23.2 ms against 29.0 with one heap per query,
+  // with 28 % longer lists; for small k the largest of four short-heap roots
+  // is far above the exact U and the lists double.  Their lists get twice
+  // the capacity: an overflowing list is re-scanned against every candidate.
+  p.pl = k >= 48;
+  const int cap = (p.pl ? 32 : 16) * k;
+  const int lo = p.pl ? 1024 : 512, hi = p.pl ? 8192 : 4096;
+  p.cap = cap < lo ? lo : (cap > hi ? hi : cap);
   return p;
 }
 
@@ -794,7 +836,8 @@ hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s)
   const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
 #define GSKNN_BF16_CASE(QB_, W_)                                                       \
   if (p.qb == QB_ && p.w == W_) {                                                      \
-    auto kern = knn_bf16_filter_kernel<KS, QB_, kCBb, W_>;                              \
+    auto kern = p.pl ? knn_bf16_filter_kernel<KS, QB_, kCBb, W_, true>                  \
+                     : knn_bf16_filter_kernel<KS, QB_, kCBb, W_, false>;                \
     hipError_t e = hipFuncSetAttribute((const void *)kern,                              \
                                        hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                        (int)p.lds);                                     \

@@ -1,6 +1,8 @@
 """The re-scan fallback of the two-pass forms: when a query's candidate list
 (in one candidate split) outgrows its capacity (clamp(16 k, 512, 4096)
-entries), the re-rank re-scans every candidate of the split exactly.
+entries; for the BF16 form with per-lane upper-bound heaps, k >= 48,
+clamp(32 k, 1024, 8192)), the re-rank re-scans every candidate of the split
+exactly.
 Thousands of candidates at exactly the same distance from every query
 (duplicated rows with distinct indices) all reach the k-th boundary; with
 16384 queries a launch splits the candidates at most a few ways, so every
@@ -19,16 +21,17 @@ pytestmark = pytest.mark.gpu
 NQ = 16384
 
 
-def _tied_table(d, dtype):
+def _tied_table(d, dtype, copies=3000):
     g = np.random.Generator(np.random.PCG64(606 + d))
     base = g.standard_normal(d)
-    X = np.empty((3200, d))
-    X[:3000] = base                       # 3000 copies: one distance per query
-    X[3000:] = base + 4.0 + g.standard_normal((200, d))  # farther points
+    m = copies + 200
+    X = np.empty((m, d))
+    X[:copies] = base                     # copies: one distance per query
+    X[copies:] = base + 4.0 + g.standard_normal((200, d))  # farther points
     q = g.standard_normal((NQ, d)) * 0.1 + base + 1.0
     X = np.concatenate([X, q]).astype(dtype)
-    amap = g.permutation(3200).astype(np.int32)
-    bmap = np.arange(3200, 3200 + NQ, dtype=np.int32)
+    amap = g.permutation(m).astype(np.int32)
+    bmap = np.arange(m, m + NQ, dtype=np.int32)
     rows = np.sort(g.choice(NQ, 96, replace=False))
     return X, amap, bmap, rows
 
@@ -61,12 +64,13 @@ def test_fp32_list_overflow_rescans_exactly(d):
     assert nover >= n, f"{nover} (split, query) lists overflowed for {n} queries"
 
 
-@pytest.mark.parametrize("d", [32, 128])
-def test_bf16_list_overflow_rescans_exactly(d):
+@pytest.mark.parametrize("d,k,copies", [(32, 10, 3000), (128, 10, 3000), (64, 64, 12000)])
+def test_bf16_list_overflow_rescans_exactly(d, k, copies):
+    # k = 64 takes the per-lane heaps (list capacity 2048): 12000 tied
+    # copies give each of the few candidate splits about 3000.
     import torch
-    print(f"[gpu] bf16 overflow d={d}", flush=True)
-    X, amap, bmap, rows = _tied_table(d, np.float32)
-    k = 10
+    print(f"[gpu] bf16 overflow d={d} k={k}", flush=True)
+    X, amap, bmap, rows = _tied_table(d, np.float32, copies)
     dev = torch.device("cuda")
     tX = torch.from_numpy(X).to(dev)
     prep = gsknn.bf16_prep_dev(tX)
