@@ -281,14 +281,15 @@ class RPTreeSweep:
     def _build(self, it, after=None):
         """Build tree `it` into spare buffer it % 2; on the side stream with
         prefetch (after event `after`: the buffer's last reader is done).
-        Returns (perm buffer, leaves, start event, done event)."""
+        Once the launch shape is known, the tree's work list is also made
+        here (on the host, while the current sweep runs) in pinned memory.
+        Returns (perm buffer, leaves, start event, done event, work list)."""
         import torch
         buf = self._bufs[it % 2]
         stream = self._side if self.prefetch else torch.cuda.current_stream()
         t0 = time.perf_counter()
         plan = plan_tree(self.X.shape[0], self.X.shape[1], it, self.seed, self.leaf_lo,
                          self.leaf_hi)
-        self._plan_ms = (time.perf_counter() - t0) * 1e3
         with torch.cuda.stream(stream):
             if after is not None:
                 stream.wait_event(after)
@@ -298,7 +299,11 @@ class RPTreeSweep:
             _, leaves = build_tree(self.X, it, self.seed, self.leaf_lo, self.leaf_hi, perm_out=buf,
                                    plan=plan)
             e1.record(stream)
-        return buf, leaves, e0, e1
+        items = None
+        if self.config is not None:
+            items = torch.from_numpy(make_items(leaves, self.config[0])).pin_memory()
+        self._plan_ms = (time.perf_counter() - t0) * 1e3
+        return buf, leaves, e0, e1, items
 
     def _ensure_config(self, perm, leaves):
         if self.config is None:
@@ -327,13 +332,14 @@ class RPTreeSweep:
         t0 = time.perf_counter()
         if self._pending is None:
             self._pending = self._build(self.iteration)
-        buf, leaves, et0, et1 = self._pending
+        buf, leaves, et0, et1, items = self._pending
         self._pending = None
         if self.config is None:
             torch.cuda.synchronize()  # the autotuner times launches on this tree
         self._ensure_config(buf, leaves)
         qpb = self.config[0]
-        items = torch.from_numpy(make_items(leaves, qpb)).pin_memory()
+        if items is None:
+            items = torch.from_numpy(make_items(leaves, qpb)).pin_memory()
         main.wait_event(et1)
         self.perm.copy_(buf)
         copied = torch.cuda.Event()
@@ -352,9 +358,11 @@ class RPTreeSweep:
         if self.prefetch:
             # The next tree goes to the other buffer (read by the previous
             # iteration's copy, long finished); it waits for this copy only
-            # so that the two streams never touch one buffer at once.
+            # so that the two streams never touch one buffer at once.  Its
+            # build may still run when this step returns: the next step's
+            # sweep waits for it on the device.
             self._pending = self._build(self.iteration + 1, after=copied)
-        torch.cuda.synchronize()
+        e1.synchronize()
         t1 = time.perf_counter()
         sweep_ms = e0.elapsed_time(e1)
         tree_ms = et0.elapsed_time(et1)
