@@ -515,12 +515,73 @@ def l1_norm(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+def dims_sweep(torch, dev, args):
+    """dgsknn FP64 (AUTO) at other dimensions: the config-2 mixture generator
+    at each d (N = 131072, 32 clusters, sigma 0.1), m = n = 65536 ids with
+    repeats, k = 32; rows empty, and rows carried over from a call on 8192
+    other candidates (the config-2 protocol).  HIP events around the whole
+    call (pre-pass, kernels, finalize), median of --steps calls after one
+    warm-up."""
+    import numpy as np
+
+    from rnn_amd import datasets as ds
+    from rnn_amd import gsknn
+
+    stream = torch.cuda.current_stream()
+    out = {"workload": "dims", "m": 65536, "n": 65536, "k": 32, "rows": {}}
+    for d in [int(x) for x in args.dims.split(",")]:
+        N, m, k = 131072, 65536, 32
+        X = ds.gaussian_mixture(N, d, 32, seed=1)
+        X2 = (X * X).sum(1)
+        g = np.random.Generator(np.random.PCG64(5))
+        am = torch.from_numpy(g.integers(0, N, m).astype(np.int32)).to(dev)
+        bm = torch.from_numpy(g.integers(0, N, m).astype(np.int32)).to(dev)
+        pre = torch.from_numpy(g.integers(0, N, 8192).astype(np.int32)).to(dev)
+        tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+        work = torch.empty(max(16, gsknn.work_bytes_d(m, m, d, k)), dtype=torch.uint8, device=dev)
+        D = torch.empty((m, k), dtype=torch.float64, device=dev)
+        I = torch.empty((m, k), dtype=torch.int32, device=dev)
+        row = {}
+        for carried in (False, True):
+            D.fill_(float("inf"))
+            I.fill_(-1)
+            if carried:
+                gsknn.dknn_dev(tX, tX2, pre, tX, tX2, bm, D, I, work=work, stream=stream)
+            D0, I0 = D.clone(), I.clone()
+
+            def call():
+                D.copy_(D0)
+                I.copy_(I0)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                gsknn.dknn_dev(tX, tX2, am, tX, tX2, bm, D, I, work=work, stream=stream)
+                e1.record(stream)
+                return e0, e1
+
+            call()
+            ts = []
+            for _ in range(max(1, args.steps)):
+                e0, e1 = call()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = float(np.median(ts))
+            frac = 2.0 * d * m * m / (ms * 1e-3) / FP64_MATRIX_PEAK
+            row["carried" if carried else "empty"] = {"ms": ms, "frac_fp64_peak": frac}
+            log(f"dims d={d} {'carried' if carried else 'empty'}: {ms:.2f} ms, {100 * frac:.1f} %")
+        out["rows"][str(d)] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "l1", "copy"])
+    ap.add_argument("--workload", default="c2",
+                    choices=["c1", "c2", "c3", "c4", "c5", "l1", "dims", "copy"])
+    ap.add_argument("--dims", default="32,64,96,128,160,192,256,300,600",
+                    help="dimensions of the dims workload")
     ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
     ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
@@ -559,6 +620,9 @@ def main():
         return
     if args.workload == "l1":
         l1_norm(torch, dev, args)
+        return
+    if args.workload == "dims":
+        print(json.dumps(dims_sweep(torch, dev, args)), flush=True)
         return
     if args.workload == "c1":
         print(json.dumps({"workload": "c1", **c1_small(torch, dev)}), flush=True)
