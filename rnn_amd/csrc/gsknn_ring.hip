@@ -60,43 +60,6 @@ typedef double rdbl2 __attribute__((ext_vector_type(2)));
 constexpr int kRing = 4;  // ring slots (pairs of k-steps)
 constexpr int kRingQ = 4;  // per-lane queue depth
 
-// Oracle distances of R pairs at once (candidate row c[r], query row q[r]):
-// each sum in p order from 0.0, every product and sum rounded; the R
-// independent sums are interleaved so their loads overlap (UNR dim pairs of
-// loads in flight per sum).  VEC: rows 16-byte aligned, d even.
-template <int R, bool VEC, int UNR>
-__device__ __forceinline__ void oracle_dist_batch(const double *const (&c)[R],
-                                                  const double *const (&q)[R], int d,
-                                                  const double (&cn)[R], const double (&qn)[R],
-                                                  double (&out)[R]) {
-  double s[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) s[r] = 0.0;
-  int p = 0;
-  if (VEC) {
-#pragma unroll UNR
-    for (; p + 2 <= d; p += 2) {
-      rdbl2 cv[R], qv[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        cv[r] = *(const rdbl2 *)(c[r] + p);
-        qv[r] = *(const rdbl2 *)(q[r] + p);
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        s[r] = xadd(s[r], xmul(cv[r].x, qv[r].x));
-        s[r] = xadd(s[r], xmul(cv[r].y, qv[r].y));
-      }
-    }
-  }
-  for (; p < d; ++p) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) s[r] = xadd(s[r], xmul(c[r][p], q[r][p]));
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) out[r] = xadd(xadd(xmul(s[r], -2.0), cn[r]), qn[r]);
-}
-
 template <int KP, int QB, int CB, int WPE, bool VEC, bool TWO>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void knn_mfma_f64_ring_kernel(
     KnnArgs<double> a) {
