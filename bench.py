@@ -460,6 +460,54 @@ def c1_small(torch, dev, reps=200):
 FP64_VALU_OPS_PEAK = 39.3e12  # 78.6 TFLOP/s FP64 vector counts an FMA as 2
 
 
+def c1_cpu_baseline(seconds=1.0):
+    """The small drop-in workload's call pair on the host: the reference's
+    own optimized dgsknn_var1 (oracle/_ref/libgsknn_opt.so) when built, else
+    the repository's C restatement of the oracle; us per call, median of 3
+    runs of enough call pairs to take about `seconds`, at the host's thread
+    share (at most 16) and at 1 thread."""
+    import numpy as np
+
+    from oracle import oracle
+    from rnn_amd import datasets as ds
+
+    p = ds.config1()
+    have_ref = oracle.ref_opt_lib() is not None
+    amaps = [p.amap, p.extra_amaps[0]]
+
+    def pairs_time(reps, threads):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            D = np.full((p.n, p.k), np.finfo(np.float64).max if have_ref else np.inf)
+            I = np.full((p.n, p.k), -1, dtype=np.int32)
+            for am in amaps:
+                if have_ref:
+                    oracle.ref_opt_gsknn(p.XA, p.XA2, am, p.XB, p.XB2, p.bmap, D, I,
+                                         threads=threads)
+                else:
+                    oracle.gsknn(p.XA, p.XA2, am, p.XB, p.XB2, p.bmap, D, I, threads=threads)
+        return time.perf_counter() - t0
+
+    def per_call_us(threads):
+        reps = 1
+        while pairs_time(reps, threads) < seconds / 2 and reps < 4096:
+            reps *= 2
+        times = sorted(pairs_time(reps, threads) for _ in range(3))
+        return times[1] / (2 * reps) * 1e6, reps
+
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, nproc or 1))
+    us, reps = per_call_us(threads)
+    us1, reps1 = per_call_us(1)
+    what = ("oracle/_ref/libgsknn_opt.so (the reference's dgsknn_var1 with its AVX kernels)"
+            if have_ref else "oracle/gsknn_oracle.c -O3 -fopenmp")
+    return {"us_per_call": us, "pairs_per_s": float(p.m) * p.n / (us * 1e-6), "cores": threads,
+            "kind": "reference" if have_ref else "port",
+            "sample": f"{reps} call pairs per run, median of 3 runs, {what}",
+            "single_thread": {"us_per_call": us1, "cores": 1,
+                              "sample": f"{reps1} call pairs per run, median of 3 runs"}}
+
+
 def l1_norm(torch, dev, args):
     """L1 (KNN_1NORM) kNN on the config-2 point set: N = 131072, d = 64,
     m = n = 65536, k = 32, empty heaps; FP64, then the same points in FP32.
@@ -625,7 +673,10 @@ def main():
         print(json.dumps(dims_sweep(torch, dev, args)), flush=True)
         return
     if args.workload == "c1":
-        print(json.dumps({"workload": "c1", **c1_small(torch, dev)}), flush=True)
+        out = {"workload": "c1", **c1_small(torch, dev)}
+        if not args.no_cpu:
+            out["cpu_baseline"] = c1_cpu_baseline()
+        print(json.dumps(out), flush=True)
         return
     t0 = time.perf_counter()
     p = ds.config2()
@@ -761,6 +812,8 @@ def main():
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p)
         log(f"cpu baseline {out['cpu_baseline']['value']:.3e} pairs/s")
+        out["c1"]["cpu_baseline"] = c1_cpu_baseline()
+        log(f"c1 cpu baseline {out['c1']['cpu_baseline']['us_per_call']:.1f} us per call")
     print(json.dumps(out), flush=True)
 
 

@@ -1,5 +1,7 @@
 """Serving loop: each request's result is the same alone and in the mix, and
-matches the oracle."""
+every request's rows match the oracle (FP32 mode: bitwise; BF16 mode, FP64
+direct differences: indices exact, distances within 1e-12 at the norm
+scale)."""
 import numpy as np
 import pytest
 
@@ -42,9 +44,9 @@ def test_alone_equals_in_mix(mode):
         Da, Ia = alone.D.numpy(), alone.I.numpy()
         Dm, Im = got[r.rid]
         assert np.array_equal(Ia, Im) and np.array_equal(Da.view(np.uint8), Dm.view(np.uint8))
-    # oracle on two requests
+    # every request against the oracle
     amap = np.arange(N, dtype=np.int32)
-    for r in reqs[:2]:
+    for r in reqs:
         q = r.queries.shape[0]
         bmap = np.arange(q, dtype=np.int32)
         Dm, Im = got[r.rid]
