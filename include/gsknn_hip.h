@@ -27,7 +27,9 @@ extern "C" {
 
 /* Contraction selection.  Every mode returns the reference oracle's
  * distance bits and the canonical (distance, lower index) neighbour set.
- * FP64: AUTO = MFMA when d >= 32, exact VALU below; EXACT = VALU with the
+ * FP64: AUTO = MFMA when d >= 32, and below it (d >= 4) for calls large
+ * enough that MFMA measured faster (calls with m <= 2048 and k <= 64 stay
+ * exact; d = 8 and 16 from 2^26 pairs, other d from 2^22); EXACT = VALU with the
  * oracle's arithmetic; MFMA = FP64 MFMA wherever a per-wave form fits: for
  * d <= 64 the form that streams candidates into registers per wave, above
  * it the operand-ring form (query operands in registers up to d = 256,

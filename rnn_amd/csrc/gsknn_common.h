@@ -44,7 +44,7 @@ struct KnnArgs {
 };
 
 enum Mode {
-  MODE_AUTO = 0,   // FP64: MFMA when d >= 32, else exact VALU
+  MODE_AUTO = 0,   // FP64: MFMA when d >= 32 or the call is large (auto_mfma), else exact VALU
   MODE_EXACT = 1,  // VALU, the reference oracle's arithmetic (bit-exact)
   MODE_MFMA = 2,   // FP64 MFMA contraction, the direct form wherever it fits
   MODE_MFMA_TILE = 3,  // FP64 MFMA, the LDS-tile kernel only (FP32: as MODE_MFMA)

@@ -1556,13 +1556,30 @@ int mfma_direct_blocks_per_cu(int ks, bool line) {
   return b;
 }
 
+// AUTO's choice of the FP64 MFMA forms (every form returns the oracle's
+// rows; this is speed only).  Measured with empty rows: MFMA for d >= 32;
+// below, exact calls small enough for the one-wave-per-query kernel stay
+// exact (m = n = 2048, d = 16: 0.10 against 0.35 ms); the scalar-candidate
+// dims 8 and 16 stay exact up to 2^26 pairs (m = n = 4097, d = 16: 0.20
+// against 0.34 ms; 16384^2: 2.31 against 1.49 ms), the other dims from 4 up
+// take MFMA from 2^22 pairs (4097^2, d = 24: 0.51 against 0.35 ms; 65536^2,
+// d = 4 / 8: 12.7 / 13.1 against 5.9 / 6.1 ms).
+bool auto_mfma(int m, int n, int d, int k) {
+  if (d >= 32) return true;
+  if (d < 4) return false;
+  if (m <= 64 * kSmallMaxP && k <= kSmallMaxK) return false;
+  const double pairs = (double)m * (double)n;
+  return pairs >= ((d == 8 || d == 16) ? 67108864.0 : 4194304.0);
+}
+
 Plan make_plan(int m, int n, int d, int k, int elem, int mode) {
   Plan p;
   const bool f64 = elem == 8;
   const int ks = f64 ? mfma_ks(d) : 0;
   // Above d = 256 only the operand-ring kernel has an MFMA form.
   const bool mfma = f64 && (ks > 0 || (ring_supported(d) && mode != MODE_MFMA_TILE)) &&
-                    (mode == MODE_MFMA || mode == MODE_MFMA_TILE || (mode == MODE_AUTO && d >= 32));
+                    (mode == MODE_MFMA || mode == MODE_MFMA_TILE ||
+                     (mode == MODE_AUTO && auto_mfma(m, n, d, k)));
   int tile_c = 0;
   if (mfma) {
     p.kind = 1;
