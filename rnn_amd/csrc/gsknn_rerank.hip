@@ -28,7 +28,7 @@
 // Pass 1 (knn_bf16_filter_kernel): per query, U = k-th smallest of
 // {upper bounds of all candidates} U {resident heap distances}; U bounds the
 // final k-th distance from above.  Every candidate whose lower bound is <= U
-// at the time it is seen is appended to the query's list (position, lower).
+// at the time it is seen is appended to the query's list (id, lower, upper).
 // Pass 2 (knn_rerank_kernel): one wave per query reads its list 64 entries
 // at a time, drops entries with lower > final U or above the row's current
 // k-th distance (which falls as entries are merged), computes sum_p (a_p - b_p)^2
@@ -135,7 +135,8 @@ struct Bf16Args {
   double *D;
   int *I;
   const int *hrow;
-  int2 *lst;    // [nsplit][n][cap] (candidate position, lower bound bits)
+  int2 *lst;    // [nsplit][n][cap] (candidate id, lower bound bits)
+  float *lstu;  // [nsplit][n][cap] upper bound of the same entry
   int *cnt;     // [nsplit][n]
   float *U;     // [nsplit][n]
   int *nover;   // queries whose list overflowed
@@ -147,7 +148,7 @@ struct Bf16Args {
 __device__ __forceinline__ int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
 
 struct FilterLayout {
-  size_t tA, nc, xn, en, heap, cnt, bytes;
+  size_t tA, nc, xn, en, id, heap, cnt, bytes;
   __host__ __device__ FilterLayout(int KS, int CB, int Q, int k) {
     const size_t tile = (size_t)KS * CB * 64 * 8;  // bf16 elements
     const size_t C = 16 * (size_t)CB;
@@ -155,7 +156,8 @@ struct FilterLayout {
     nc = al16r(tA + tile * 2);
     xn = al16r(nc + C * 4);
     en = al16r(xn + C * 4);
-    heap = al16r(en + C * 4);
+    id = al16r(en + C * 4);
+    heap = al16r(id + C * 4);
     cnt = al16r(heap + (size_t)4 * Q * ((k + 3) / 4) * 4);
     bytes = al16r(cnt + (size_t)Q * 4);
   }
@@ -211,6 +213,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   float *tnc = (float *)(smem + L.nc);
   float *txn = (float *)(smem + L.xn);
   float *ten = (float *)(smem + L.en);
+  int *tcid = (int *)(smem + L.id);
   float *hv = (float *)(smem + L.heap);
   int *hc = (int *)(smem + L.cnt);
 
@@ -306,8 +309,12 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
   constexpr int NPW = (CB + W - 1) / W;  // norm blocks per wave
   uint4 stage[BPW];
   float st_n[NPW], st_x[NPW], st_e[NPW];
+  int st_i[NPW];
 #pragma unroll
-  for (int r = 0; r < NPW; ++r) st_n[r] = st_x[r] = st_e[r] = 0.f;
+  for (int r = 0; r < NPW; ++r) {
+    st_n[r] = st_x[r] = st_e[r] = 0.f;
+    st_i[r] = -1;
+  }
   // Ids of this thread's candidates: of the tile being staged (sid) and of
   // the tile after it (nid, requested a tile ahead).
   typedef int idvec __attribute__((ext_vector_type(CB)));  // keeps selections in registers
@@ -345,6 +352,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
           for (int cb = 1; cb < CB; ++cb)
             if (nb == cb) id = sid[cb];
           st_n[r] = (float)n64A[id];
+          st_i[r] = id;
           st_x[r] = xnA[id];
           st_e[r] = enA[id];
         }
@@ -366,6 +374,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
           const int slot = nb * 16 + si;
           const bool ok = cbeg + t * C + slot < cend;
           tnc[slot] = ok ? st_n[r] : __builtin_inff();
+          tcid[slot] = st_i[r];
           txn[slot] = ok ? st_x[r] : 0.0f;
           ten[slot] = ok ? st_e[r] : 0.0f;
         }
@@ -403,7 +412,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     }
   };
 
-  // Per-lane queue: lower, upper, candidate position, query block.
+  // Per-lane queue: lower, upper, candidate id, query block.
   float pq_lo[QR], pq_up[QR];
   int pq_ps[QR], pq_q[QR];
   int pq_n = 0;
@@ -421,8 +430,11 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     const float Uq = PL ? uq[qb] : hv[ql[qb]];
     if (lower <= Uq) {
       const int at = PL ? atomicAdd(&hc[ql[qb]], 1) : hc[ql[qb]]++;
-      if (at < a.cap)
-        a.lst[(sbase + qbase + ql[qb]) * a.cap + at] = make_int2(ps, __float_as_int(lower));
+      if (at < a.cap) {
+        const size_t li = (sbase + qbase + ql[qb]) * a.cap + at;
+        a.lst[li] = make_int2(ps, __float_as_int(lower));
+        a.lstu[li] = upper;
+      }
     }
     if constexpr (PL) {
       const int hs = hslot(qb);
@@ -531,7 +543,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
               const float dist = tt + nq[qb];
               const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
                                       1.0009765625f + 1e-30f;
-              apply(qb, dist - delta, dist + delta, cbeg + t * C + c);
+              apply(qb, dist - delta, dist + delta, tcid[c]);
             }
           }
         });
@@ -557,7 +569,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
               if (pq_n == r) {
                 pq_lo[r] = dist - delta;
                 pq_up[r] = dist + delta;
-                pq_ps[r] = cbeg + t * C + c;
+                pq_ps[r] = tcid[c];
                 pq_q[r] = qb;
               }
             ++pq_n;
@@ -609,46 +621,114 @@ __device__ void heap_sift_d(double *hd, int *hi, int s, int k, double xd, int xi
   hi[s] = xi;
 }
 
-// FP64 direct difference sum_p (a_p - b_p)^2 of one candidate, computed by a
-// 16-lane group: lane i of the group sums dims i, i + 16, ... in order (FMA
-// chain), then a fixed tree over the group (16 -> 1) gives the total in the
-// group's first lane.  The order is the same for every call, so a pair's
-// distance does not depend on the list, the split or the batch it comes in.
-constexpr int kRerankDims = 256;  // d limit of the BF16 form
-constexpr int kDimsPerLane = kRerankDims / 16;
+// FP64 direct differences sum_p (a_p - b_p)^2 of up to kRowBatch candidate
+// rows for one query, computed by the whole wave: lane l holds query dims
+// 4 l .. 4 l + 3 (qv, zero beyond d) and reads the same dims of each row with
+// one 16-byte load (scalar loads when d is not a multiple of 4), sums its four
+// terms in order, and a fixed butterfly over the 64 lanes adds the partial
+// sums (the same tree for every row).  A pair's distance therefore does not
+// depend on the list, the batch or the call it comes in.  Row ids are in
+// lanes 0 .. nb - 1 of myid; the distance of row b is returned in lanes
+// kRowLanes b .. kRowLanes (b + 1) - 1.  d <= 256.
+constexpr int kRowBatch = 4;
+constexpr int kRowLanes = 64 / kRowBatch;  // lanes holding each row's distance
 
-__device__ __forceinline__ double group16_sum(double v) {
-  for (int off = 8; off >= 1; off >>= 1) v += __shfl_down(v, off, 16);
-  return v;
+__device__ double wave_row_dists(const float *XA, int d, int myid, int nb, const double qv[4]) {
+  const int lane = threadIdx.x & 63;
+  const bool vec4 = (d & 3) == 0;  // rows 16-byte aligned
+  double v[kRowBatch];
+#pragma unroll
+  for (int b = 0; b < kRowBatch; ++b) {
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int id = __builtin_amdgcn_readlane(myid, b);
+    const float *xa = XA + (size_t)id * d + 4 * lane;
+    if (b < nb) {
+      if (vec4) {
+        if (4 * lane < d) x = *(const float4 *)xa;
+      } else {
+        if (4 * lane < d) x.x = xa[0];
+        if (4 * lane + 1 < d) x.y = xa[1];
+        if (4 * lane + 2 < d) x.z = xa[2];
+        if (4 * lane + 3 < d) x.w = xa[3];
+      }
+    }
+    double df = (double)x.x - qv[0];
+    double sacc = df * df;
+    df = (double)x.y - qv[1];
+    sacc = __builtin_fma(df, df, sacc);
+    df = (double)x.z - qv[2];
+    sacc = __builtin_fma(df, df, sacc);
+    df = (double)x.w - qv[3];
+    v[b] = __builtin_fma(df, df, sacc);
+  }
+  // Lanes differing in bit 5, then 4 (then 3, for 8 rows) exchange half of
+  // their values; the remaining lane bits add.  Row b ends in the lanes
+  // whose top log2(kRowBatch) bits are b.
+  const bool b5 = lane & 32, b4 = lane & 16;
+  double w1;
+  if constexpr (kRowBatch == 8) {
+    const bool b3 = lane & 8;
+    double w4[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double send = b5 ? v[c] : v[c + 4];
+      w4[c] = (b5 ? v[c + 4] : v[c]) + __shfl_xor(send, 32);
+    }
+    double w2[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double send = b4 ? w4[c] : w4[c + 2];
+      w2[c] = (b4 ? w4[c + 2] : w4[c]) + __shfl_xor(send, 16);
+    }
+    w1 = (b3 ? w2[1] : w2[0]) + __shfl_xor(b3 ? w2[0] : w2[1], 8);
+  } else {
+    static_assert(kRowBatch == 4, "4 or 8 rows per batch");
+    double w2[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double send = b5 ? v[c] : v[c + 2];
+      w2[c] = (b5 ? v[c + 2] : v[c]) + __shfl_xor(send, 32);
+    }
+    w1 = (b4 ? w2[1] : w2[0]) + __shfl_xor(b4 ? w2[0] : w2[1], 16);
+    w1 += __shfl_xor(w1, 8);
+  }
+  w1 += __shfl_xor(w1, 4);
+  w1 += __shfl_xor(w1, 2);
+  w1 += __shfl_xor(w1, 1);
+  return w1;
 }
 
-__global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// This lane's query dims 4 lane .. 4 lane + 3 for wave_row_dists.
+__device__ __forceinline__ void load_query4(const float *xq, int d, double qv[4]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int p = 4 * lane + i;
+    qv[i] = p < d ? (double)xq[p] : 0.0;
+  }
+}
+
+// Heap form of pass 2 for query j: the row as a max-heap in LDS (hd, hi: k
+// entries), list entries in list order, dropped when their lower bound
+// exceeds the filter's U or the row's current k-th distance; an overflowed
+// list re-scans every candidate of its split.  Used for overflowed lists and
+// for k above the selection form's limit.
+__device__ void rerank_heap_path(const Bf16Args &a, int j, double *hd, int *hi) {
   const int k = a.k, d = a.d;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int g = lane >> 4, gl = lane & 15;
-  const int j = blockIdx.x * 4 + w;
-  double *hd = (double *)smem + (size_t)w * k;
-  int *hi = (int *)(smem + al16r(sizeof(double) * 4 * (size_t)k)) + (size_t)w * k;
-  if (j >= a.n) return;  // whole wave leaves together; no block barrier below
+  const int lane = threadIdx.x & 63;
   const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
-  if (lane == 0) {
-    for (int t = 0; t < k; ++t) {
-      hd[t] = a.D[row + t];
-      hi[t] = a.I[row + t];
-    }
-    for (int s = k / 2 - 1; s >= 0; --s) heap_sift_d(hd, hi, s, k, hd[s], hi[s]);
+  for (int t = lane; t < k; t += 64) {
+    hd[t] = a.D[row + t];
+    hi[t] = a.I[row + t];
   }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  // This lane's query dims gl, gl + 16, ... (the same in every group).
-  const float *xq = a.XB + (size_t)a.bmap[j] * d;
-  double qv[kDimsPerLane];
-#pragma unroll
-  for (int i = 0; i < kDimsPerLane; ++i) {
-    const int p = gl + 16 * i;
-    qv[i] = p < d ? (double)xq[p] : 0.0;
-  }
+  if (lane == 0)
+    for (int s = k / 2 - 1; s >= 0; --s) heap_sift_d(hd, hi, s, k, hd[s], hi[s]);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  double qv[4];
+  load_query4(a.XB + (size_t)a.bmap[j] * d, d, qv);
   // Every split's U bounds the final k-th distance; prune with the smallest.
   float Uf = a.U[j];
   for (int sp = 1; sp < a.nsplit; ++sp) Uf = fminf(Uf, a.U[(size_t)sp * a.n + j]);
@@ -664,59 +744,41 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
     for (int base = 0; base < nscan; base += 64) {
       // One list entry (or, for an overflowed list, one candidate) per lane.
       const int e = base + lane;
-      int pos = 0;
+      int cid = 0;
       float lw = -__builtin_inff();  // the entry's lower bound (none when re-scanning)
       bool keep = e < nscan;
       if (keep) {
         if (over) {
-          pos = cbeg + e;
+          cid = a.amap[cbeg + e];
         } else {
           const int2 en = a.lst[sj * a.cap + e];
-          pos = en.x;
+          cid = en.x;
           lw = __int_as_float(en.y);
           keep = lw <= Ulim && (double)lw <= rd;
         }
       }
       unsigned long long live = __ballot(keep);
       while (live) {
-        // Up to four kept entries: group g takes the g-th lowest set bit.
-        int src[4];
-        int nb = 0;
+        // Up to kRowBatch kept entries: lane t < nb takes the t-th lowest set bit.
+        int nb = 0, mysrc = 0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          src[t] = live ? __builtin_ctzll(live) : 0;
+        for (int t = 0; t < kRowBatch; ++t) {
           if (live) {
+            if (lane == t) mysrc = __builtin_ctzll(live);
             live &= live - 1;
             ++nb;
           }
         }
-        int my = src[0];
-#pragma unroll
-        for (int t = 1; t < 4; ++t)
-          if (g == t) my = src[t];
-        const int mpos = __shfl(pos, my);
-        const bool act = g < nb;
-        const int id = act ? a.amap[mpos] : 0;
-        double s = 0.0;
-        if (act) {
-          const float *xa = a.XA + (size_t)id * d;
-#pragma unroll
-          for (int i = 0; i < kDimsPerLane; ++i) {
-            const int p = gl + 16 * i;
-            if (p < d) {
-              const double t = (double)xa[p] - qv[i];
-              s = __builtin_fma(t, t, s);
-            }
-          }
-        }
-        s = group16_sum(s);
-        // Apply the group results in group order (canonical selection does
-        // not depend on it).
+        const int sid = __shfl(cid, mysrc);
+        const int myid = lane < nb ? sid : 0;
+        const double v = wave_row_dists(a.XA, d, myid, nb, qv);
+        // Apply the results in list order (canonical selection does not
+        // depend on it).
         for (int t = 0; t < nb; ++t) {
-          const double dist = __shfl(s, 16 * t);
-          const int cid = __shfl(id, 16 * t);
-          if (cless_d(dist, cid, rd, ri)) {
-            if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, cid);
+          const double dist = __shfl(v, kRowLanes * t);
+          const int tid = __shfl(myid, t);
+          if (cless_d(dist, tid, rd, ri)) {
+            if (lane == 0) heap_sift_d(hd, hi, 0, k, dist, tid);
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             rd = hd[0];
@@ -742,6 +804,332 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a) {
       a.I[row + t] = hi[k - 1 - t];
     }
   }
+}
+
+// only_over: take only the queries with an overflowed list (the selection
+// form leaves them).
+__global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a, int only_over) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k;
+  const int w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  double *hd = (double *)smem + (size_t)w * k;
+  int *hi = (int *)(smem + al16r(sizeof(double) * 4 * (size_t)k)) + (size_t)w * k;
+  if (j >= a.n) return;  // whole wave leaves together; no block barrier below
+  if (only_over) {
+    bool over = false;
+    for (int sp = 0; sp < a.nsplit; ++sp) over |= a.cnt[(size_t)sp * a.n + j] > a.cap;
+    if (!over) return;
+  }
+  rerank_heap_path(a, j, hd, hi);
+}
+
+// Selection form of pass 2 (k <= kSelMaxK), one wave per query; a query
+// with an overflowed list is left to knn_rerank_kernel:
+//  1. T = the k-th smallest of {upper bounds of the listed entries} U {the
+//     resident row's distances rounded up to FP32}, bounded from above by a
+//     radix select over the top 24 bits of the keys (three 8-bit digits,
+//     LDS histogram; the keys are staged in LDS by the first digit's pass):
+//     T is the largest value with the k-th key's 24-bit prefix, at most
+//     2^-15 relative above it.  k items have distances at or below T, so the
+//     final k-th distance is at most T: an entry whose lower bound exceeds T
+//     (with a relative slack of 2^-20 for the FP64 rounding of the re-ranked
+//     sums) cannot enter the row.  Every candidate with one of the k
+//     smallest upper bounds was listed, so T is (up to the 2^-15) the exact U
+//     of the whole candidate stream.
+//  2. The entries at or below T join the resident row in an LDS pool of
+//     64 NS entries, then get their FP64 direct-difference distances
+//     kRowBatch at a time (wave_row_dists).
+//  3. The pool's k smallest entries in (distance, index) order are found by
+//     ranking every entry against the pool (equal pairs, duplicates of one
+//     candidate, break by pool slot); a full pool is cut back to k the same
+//     way.  The row is written in descending order (a valid max-heap).
+// List loads go eight per lane at a time, so one memory latency covers 512
+// entries.
+constexpr int kSelMaxK = 256;
+constexpr int kSelKeys = 1024;                        // keys staged in LDS per wave
+constexpr int kSelUnroll = 8;                         // list loads per lane in flight
+
+// Pool entries per lane: room for k plus at least two windows of 64.
+constexpr int sel_slots(int k) { return (k + 128 + 63) / 64; }
+
+struct SelLayout {
+  size_t hist, keys, pd, pi, bytes;
+  int P;
+  __host__ __device__ SelLayout(int k) {
+    P = 64 * sel_slots(k);
+    hist = 0;
+    keys = al16r(256 * 4);
+    pd = al16r(keys + (size_t)kSelKeys * 4);
+    pi = al16r(pd + (size_t)P * 8);
+    bytes = al16r(pi + (size_t)P * 4);
+  }
+};
+
+// Order-preserving map of FP32 bit patterns to unsigned keys and back.
+__device__ __forceinline__ unsigned fkey(float f) {
+  const unsigned b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float keyf(unsigned key) {
+  return __uint_as_float((key & 0x80000000u) ? (key & 0x7fffffffu) : ~key);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void knn_rerank_select_kernel(Bf16Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int k = a.k, d = a.d;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  const SelLayout SL(k);
+  unsigned char *wbase = smem + (size_t)w * SL.bytes;
+  unsigned *hist = (unsigned *)(wbase + SL.hist);
+  unsigned *keys = (unsigned *)(wbase + SL.keys);
+  double *pd = (double *)(wbase + SL.pd);
+  int *pi = (int *)(wbase + SL.pi);
+  if (j >= a.n) return;  // whole wave leaves together; no block barrier below
+  bool over = false;
+  int ntot = k;  // keys: the resident row, then each split's list
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const int c = a.cnt[(size_t)sp * a.n + j];
+    over |= c > a.cap;
+    ntot += c;
+  }
+  if (over) return;
+  auto wsync = []() __attribute__((always_inline)) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
+  const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  double qv[4];
+  load_query4(a.XB + (size_t)a.bmap[j] * d, d, qv);
+  for (int t = lane; t < k; t += 64) {
+    pd[t] = a.D[row + t];
+    pi[t] = a.I[row + t];
+  }
+  for (int b = lane; b < 256; b += 64) hist[b] = 0u;
+  wsync();
+
+  // 1. Radix select of the k-th smallest key.  Digit 0 reads the keys from
+  // the row and the lists and stages the first kSelKeys in LDS.
+  for (int t = lane; t < k; t += 64) {
+    const unsigned key = fkey(__double2float_ru(pd[t]));
+    if (t < kSelKeys) keys[t] = key;
+    atomicAdd(&hist[key >> 24], 1u);
+  }
+  {
+    int kbase = k;  // key index of this split's first entry
+    for (int sp = 0; sp < a.nsplit; ++sp) {
+      const size_t sj = (size_t)sp * a.n + j;
+      const int cnt = a.cnt[sj];
+      const float *up = a.lstu + sj * a.cap;
+      for (int base = 0; base < cnt; base += 64 * kSelUnroll) {
+        float v[kSelUnroll];
+#pragma unroll
+        for (int u = 0; u < kSelUnroll; ++u) {
+          const int e = base + lane + 64 * u;
+          v[u] = e < cnt ? up[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kSelUnroll; ++u) {
+          const int e = base + lane + 64 * u;
+          if (e < cnt) {
+            const unsigned key = fkey(v[u]);
+            if (kbase + e < kSelKeys) keys[kbase + e] = key;
+            atomicAdd(&hist[key >> 24], 1u);
+          }
+        }
+      }
+      kbase += cnt;
+    }
+  }
+  unsigned prefix = 0;
+  int rank = k;  // 1-based rank still to find below the current prefix
+#pragma unroll 1
+  for (int pass = 0; pass < 3; ++pass) {
+    const int sh = 24 - 8 * pass;
+    if (pass > 0) {
+      for (int b = lane; b < 256; b += 64) hist[b] = 0u;
+      wsync();
+      auto count = [&](unsigned key) __attribute__((always_inline)) {
+        if (((key >> sh) >> 8) == prefix) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+      };
+      const int nl = ntot < kSelKeys ? ntot : kSelKeys;
+      for (int e = lane; e < nl; e += 64) count(keys[e]);
+      if (ntot > kSelKeys) {
+        // Keys beyond the LDS stage are read again from the lists.
+        int kbase = k;
+        for (int sp = 0; sp < a.nsplit; ++sp) {
+          const size_t sj = (size_t)sp * a.n + j;
+          const int cnt = a.cnt[sj];
+          const float *up = a.lstu + sj * a.cap;
+          for (int e = lane; e < cnt; e += 64)
+            if (kbase + e >= kSelKeys) count(fkey(up[e]));
+          kbase += cnt;
+        }
+      }
+    }
+    wsync();
+    // Lane l holds bins 4 l .. 4 l + 3; inclusive scan over the lanes.
+    const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                   h3 = hist[4 * lane + 3];
+    const int local = (int)(h0 + h1 + h2 + h3);
+    int incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
+    }
+    const int excl = incl - local;
+    const unsigned long long hitm = __ballot(excl < rank && rank <= incl);
+    const int src = (int)__builtin_ctzll(hitm);
+    int bin = 0, before = excl;
+    {
+      int c = excl;
+      const unsigned hv4[4] = {h0, h1, h2, h3};
+      bool found = false;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!found && c + (int)hv4[t] >= rank) {
+          bin = 4 * lane + t;
+          before = c;
+          found = true;
+        }
+        c += (int)hv4[t];
+      }
+    }
+    bin = __shfl(bin, src);
+    before = __shfl(before, src);
+    prefix = (prefix << 8) | (unsigned)bin;
+    rank -= before;
+    wsync();  // histogram read before the next pass clears it
+  }
+  // The largest key with the 24-bit prefix (+inf when the prefix is
+  // +inf's: keys above it are NaNs).
+  const unsigned tkey = min((prefix << 8) | 0xFFu, 0xFF800000u);
+  const float T = keyf(tkey);
+  const float Tlim = T * (1.0f + 0x1p-20f) + 1e-30f;
+
+  int np = k;    // pool entries; slots np0 .. np - 1 still lack distances
+  int np0 = k;
+  // 2b. Distances of the pending pool slots, kRowBatch rows at a time.
+  auto distances = [&]() {
+    wsync();
+    for (int s0 = np0; s0 < np; s0 += kRowBatch) {
+      const int nb = min(kRowBatch, np - s0);
+      const int myid = lane < nb ? pi[s0 + lane] : 0;
+      const double v = wave_row_dists(a.XA, d, myid, nb, qv);
+      if (lane % kRowLanes == 0 && lane / kRowLanes < nb) pd[s0 + lane / kRowLanes] = v;
+    }
+    np0 = np;
+  };
+  // 3. Keep the k smallest pool entries; the last round writes the row.
+  auto select = [&](bool last) {
+    distances();
+    wsync();
+    double od[NS];
+    int oi[NS], orank[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int e = lane + 64 * s;
+      od[s] = 0.0;
+      oi[s] = 0;
+      orank[s] = 0;
+      if (e < np) {
+        od[s] = pd[e];
+        oi[s] = pi[e];
+      }
+    }
+    const int ns = (np + 63) >> 6;  // occupied slots per lane
+    for (int jj = 0; jj < np; ++jj) {
+      const double dj = pd[jj];
+      const int ij = pi[jj];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (s < ns) {
+          const int e = lane + 64 * s;
+          const bool lt = dj < od[s] || (dj == od[s] && (ij < oi[s] || (ij == oi[s] && jj < e)));
+          orank[s] += lt ? 1 : 0;
+        }
+      }
+    }
+    wsync();  // every rank computed before the pool is rewritten
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int e = lane + 64 * s;
+      if (e < np && orank[s] < k) {
+        if (last) {
+          a.D[row + k - 1 - orank[s]] = od[s];
+          a.I[row + k - 1 - orank[s]] = oi[s];
+        } else {
+          pd[orank[s]] = od[s];
+          pi[orank[s]] = oi[s];
+        }
+      }
+    }
+    np = np0 = k;
+    wsync();
+  };
+
+  // 2a. Compact the entries at or below T into the pool.
+  // A batch whose survivors would overflow the pool is taken one window at a
+  // time, cutting the pool back to k before a window that does not fit.
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const size_t sj = (size_t)sp * a.n + j;
+    const int cnt = a.cnt[sj];
+    const int2 *ls = a.lst + sj * a.cap;
+    for (int base = 0; base < cnt; base += 64 * kSelUnroll) {
+      int2 en[kSelUnroll];
+#pragma unroll
+      for (int u = 0; u < kSelUnroll; ++u) {
+        const int e = base + lane + 64 * u;
+        en[u] = e < cnt ? ls[e] : make_int2(0, __float_as_int(__builtin_inff()));
+      }
+      bool keep[kSelUnroll];
+      int id[kSelUnroll];
+      int nkeep = 0;
+#pragma unroll
+      for (int u = 0; u < kSelUnroll; ++u) {
+        keep[u] = base + lane + 64 * u < cnt && __int_as_float(en[u].y) <= Tlim;
+        id[u] = en[u].x;
+        nkeep += __popcll(__ballot(keep[u]));
+      }
+      if (np + nkeep <= SL.P) {
+#pragma unroll
+        for (int u = 0; u < kSelUnroll; ++u) {
+          const unsigned long long live = __ballot(keep[u]);
+          if (keep[u]) {
+            const int at = np + __builtin_amdgcn_mbcnt_hi((unsigned)(live >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)live, 0u));
+            pi[at] = id[u];
+          }
+          np += __popcll(live);
+        }
+      } else {
+#pragma unroll 1
+        for (int u = 0; u < kSelUnroll; ++u) {
+          const int e = base + lane + 64 * u;
+          bool kp = false;
+          int cid = 0;
+          if (e < cnt) {
+            const int2 en1 = ls[e];
+            kp = __int_as_float(en1.y) <= Tlim;
+            cid = en1.x;
+          }
+          const unsigned long long live = __ballot(kp);
+          if (live == 0ull) continue;
+          if (np + __popcll(live) > SL.P) select(false);
+          if (kp) {
+            const int at = np + __builtin_amdgcn_mbcnt_hi((unsigned)(live >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)live, 0u));
+            pi[at] = cid;
+          }
+          np += __popcll(live);
+        }
+      }
+    }
+  }
+  select(true);
 }
 
 namespace {
@@ -817,7 +1205,7 @@ Bf16Plan plan_bf16(int d, int k) {
 }
 
 size_t bf16_work_layout(int n, int cap, int nsplit, size_t *o_lst, size_t *o_cnt, size_t *o_u,
-                        size_t *o_over) {
+                        size_t *o_over, size_t *o_lstu = nullptr) {
   size_t off = 0;
   const size_t rows = (size_t)n * nsplit;
   *o_lst = off;
@@ -828,7 +1216,20 @@ size_t bf16_work_layout(int n, int cap, int nsplit, size_t *o_lst, size_t *o_cnt
   off = al16r(off + rows * 4);
   *o_over = off;
   off = al16r(off + 16);
+  if (o_lstu) *o_lstu = off;
+  off = al16r(off + rows * cap * sizeof(float));
   return off;
+}
+
+using SelKernel = void (*)(Bf16Args);
+SelKernel sel_kernel(int ns) {
+  switch (ns) {
+    case 3: return knn_rerank_select_kernel<3>;
+    case 4: return knn_rerank_select_kernel<4>;
+    case 5: return knn_rerank_select_kernel<5>;
+    case 6: return knn_rerank_select_kernel<6>;
+    default: return knn_rerank_select_kernel<7>;
+  }
 }
 
 template <int KS>
@@ -883,8 +1284,8 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   plan_splits(p, m, n);
   const size_t rl = al16r(sizeof(double) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;
   if (rl > kLds) return hipErrorNotSupported;
-  size_t o_lst, o_cnt, o_u, o_over;
-  const size_t need = bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over);
+  size_t o_lst, o_cnt, o_u, o_over, o_lstu;
+  const size_t need = bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over, &o_lstu);
   if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
   Bf16Args a{};
   a.m = m; a.n = n; a.d = d; a.k = k; a.ldk = ldk; a.cap = p.cap;
@@ -896,6 +1297,7 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   a.amap = amap; a.bmap = bmap;
   a.D = D; a.I = I; a.hrow = hrow;
   a.lst = (int2 *)((char *)work + o_lst);
+  a.lstu = (float *)((char *)work + o_lstu);
   a.cnt = (int *)((char *)work + o_cnt);
   a.U = (float *)((char *)work + o_u);
   a.nover = (int *)((char *)work + o_over);
@@ -910,10 +1312,22 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
     default: return hipErrorNotSupported;
   }
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)knn_rerank_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)rl);
+  // Selection form, then the heap form for the queries it leaves
+  // (overflowed lists), or the heap form for every query above kSelMaxK.
+  const bool sel = k <= kSelMaxK;
+  if (sel) {
+    const size_t sl = 4 * SelLayout(k).bytes;
+    auto kern = sel_kernel(sel_slots(k));
+    e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((n + 3) / 4), dim3(256), sl, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  e = hipFuncSetAttribute((const void *)knn_rerank_kernel,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rl);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(knn_rerank_kernel, dim3((n + 3) / 4), dim3(256), rl, s, a);
+  hipLaunchKernelGGL(knn_rerank_kernel, dim3((n + 3) / 4), dim3(256), rl, s, a, sel ? 1 : 0);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (overflow_out)
