@@ -56,13 +56,19 @@ __device__ __forceinline__ bool cless_d(double da, int ia, double db, int ib) {
 
 }  // namespace
 
+// BF16 row length: d rounded up to 32 KS dims, KS = 1, 2, 4 or 8 (the
+// k-steps the filter kernels are built for), zero padded.
+__host__ __device__ constexpr int bf16_row_dims(int d) {
+  return d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
+}
+
 // Prepared-table layout: bf16 rows [N][DP] | n64 [N] | xn [N] | en [N] |
 // maxima {xn, en, n32} as 3 floats.
 struct PrepLayout {
   size_t xb, n64, xn, en, mx, bytes;
   int DP;
   __host__ __device__ PrepLayout(int N, int d) {
-    DP = (d + 31) / 32 * 32;
+    DP = bf16_row_dims(d);
     xb = 0;
     n64 = al16r(xb + (size_t)N * DP * 2);
     xn = al16r(n64 + (size_t)N * 8);
@@ -135,12 +141,22 @@ struct Bf16Args {
   double *D;
   int *I;
   const int *hrow;
+  const unsigned char *panel;  // register form: candidates gathered in position order
+  int G;        // lists per (split, query): the cap entries split into G segments
   int2 *lst;    // [nsplit][n][cap] (candidate id, lower bound bits)
   float *lstu;  // [nsplit][n][cap] upper bound of the same entry
-  int *cnt;     // [nsplit][n]
+  int *cnt;     // [nsplit][G][n] entries appended to each list
   float *U;     // [nsplit][n]
-  int *nover;   // queries whose list overflowed
+  int *nover;   // lists that overflowed
 };
+
+// List gg of split sp of query j: its count and its first entry.
+__device__ __forceinline__ int list_count(const Bf16Args &a, int sp, int gg, int j) {
+  return a.cnt[((size_t)sp * a.G + gg) * a.n + j];
+}
+__device__ __forceinline__ size_t list_base(const Bf16Args &a, int sp, int gg, int j) {
+  return ((size_t)sp * a.n + j) * a.cap + (size_t)gg * (a.cap / a.G);
+}
 
 // Lane l of v_mfma_f32_16x16x32_bf16 holds 8 consecutive k of row l%16 of A
 // (and of column l%16 of B) starting at k = 8*(l/16); result entry v of lane l
@@ -593,9 +609,327 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
       const int j = qbase + ql[qb];
       if (j < a.n) {
         a.U[sbase + j] = uf[qb];
-        a.cnt[sbase + j] = hc[ql[qb]];
+        a.cnt[sbase + j] = hc[ql[qb]];  // G = 1: [split][n]
         if (hc[ql[qb]] > a.cap) atomicAdd(a.nover, 1);
       }
+    }
+  }
+}
+
+// Register form of pass 1 (k from 48 to 4 RK): the same contraction, bounds
+// and lists, with
+//  * upper-bound sets in registers: lane group g keeps, for each of its
+//    queries, its kg = (k + 3 - g) / 4 smallest upper bounds (the resident
+//    row's entries g, g + 4, ... and the candidates its lanes see) as a
+//    sorted array of RK floats, padded below with -inf; an insertion is a
+//    branch-free min/max shift (a value at or above the largest changes
+//    nothing), and U = the largest of the four groups' largest (the four
+//    sets hold k values, all at or below it);
+//  * one list segment per (query, lane group): the group appends to its own
+//    segment of cap / 4 entries with a counter in a register (no atomics);
+//  * the candidates gathered beforehand into a panel in candidate order
+//    (bf16_gather_kernel: BF16 rows and a record {nc, xn, en, id} per
+//    position), so every load of the tile loop has an affine address;
+//  * candidate tiles loaded straight into LDS (global_load_lds_dwordx4,
+//    lane l's 16 bytes at the block's base + 16 l, the fragment layout the
+//    MFMA reads), two tile buffers, one barrier per tile: tile t + 1 loads
+//    while tile t is computed;
+//  * survivors applied in the tile they pass; lane l of every wave holds the
+//    bound terms and id of the tile's candidate l in registers and survivors
+//    fetch them by lane shuffles (no LDS reads after the next tile's loads
+//    are issued, which the compiler would make wait for those loads).
+// Candidate panel of the register form: BF16 rows [m][DP] in candidate
+// position order, then a record {nc, xn, en, id} per position (nc = the FP64
+// squared norm rounded to FP32, id = amap[position]).
+struct PanelLayout {
+  size_t rows, rec, bytes;
+  __host__ __device__ PanelLayout(int m, int DP) {
+    rows = 0;
+    rec = al16r((size_t)m * DP * 2);
+    bytes = al16r(rec + (size_t)m * 16);
+  }
+};
+
+// One wave per candidate position: 16 bytes of the row per lane.
+__global__ __launch_bounds__(256) void bf16_gather_kernel(int m, int NA, int d, const int *amap,
+                                                          const unsigned char *prepA,
+                                                          unsigned char *panel) {
+  const PrepLayout PA(NA, d);
+  const int DP = PA.DP;
+  const PanelLayout P(m, DP);
+  const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (pos >= m) return;
+  const int id = amap[pos];
+  const uint4 *src = (const uint4 *)(prepA + PA.xb + (size_t)id * DP * 2);
+  uint4 *dst = (uint4 *)(panel + P.rows + (size_t)pos * DP * 2);
+  for (int c = lane; c < DP / 8; c += 64) dst[c] = src[c];
+  if (lane == 0) {
+    const float nc = (float)((const double *)(prepA + PA.n64))[id];
+    const float xn = ((const float *)(prepA + PA.xn))[id];
+    const float en = ((const float *)(prepA + PA.en))[id];
+    ((float4 *)(panel + P.rec))[pos] = make_float4(nc, xn, en, __int_as_float(id));
+  }
+}
+
+// Per buffer: the tile and the candidates' FP32 norms (read by every pair).
+struct Filter2Layout {
+  size_t tile, nc, buf, bytes;
+  __host__ __device__ Filter2Layout(int KS, int CB) {
+    const size_t C = 16 * (size_t)CB;
+    tile = 0;
+    nc = al16r(tile + (size_t)KS * CB * 64 * 16);
+    buf = al16r(nc + C * 4);  // one buffer; two are used
+    bytes = 2 * buf;
+  }
+};
+
+template <int KS, int QB, int CB, int W, int RK>
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
+  constexpr int DP = 32 * KS;
+  constexpr int BLK = KS * CB;  // 64 x 8 bf16 fragment blocks per tile
+  static_assert(BLK % W == 0, "fragment blocks per wave");
+  static_assert(C == 64, "one candidate per lane");
+  constexpr int BPW = BLK / W;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Filter2Layout L(KS, CB);
+  const int k = a.k;
+
+  const PrepLayout PA(a.NA, a.d), PB(a.NB, a.d);
+  const __bf16 *xbB = (const __bf16 *)(a.prepB + PB.xb);
+  const double *n64B = (const double *)(a.prepB + PB.n64);
+  const float *xnB = (const float *)(a.prepB + PB.xn);
+  const float *enB = (const float *)(a.prepB + PB.en);
+  const float *mxA = (const float *)(a.prepA + PA.mx);
+  const PanelLayout PL(a.m, DP);
+  const __bf16 *xg = (const __bf16 *)(a.panel + PL.rows);
+  const float4 *grec = (const float4 *)(a.panel + PL.rec);
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4, si = lane & 15;
+  const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const int capg = a.cap / 4;
+
+  const float u = kU32;
+  const float g = (float)((a.d + 1) * 2.0 * 0x1p-24 / (1.0 - (a.d + 1) * 2.0 * 0x1p-24));
+  const float cxx = 2.0f * g + 8.0f * u;
+  bf16x8 bq[QB][KS];
+  float nq[QB], al[QB], be[QB], cq[QB], dmx[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + si;
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const int row = a.bmap[qv[qb] ? j : a.n - 1];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const bf16x8 v = *(const bf16x8 *)(xbB + (size_t)row * DP + s * 32 + 8 * grp);
+      bq[qb][s] = qv[qb] ? v : bf16x8{};
+    }
+    const float xq = xnB[row];
+    const float eq = enB[row];
+    nq[qb] = (float)n64B[row];
+    al[qb] = 2.0f * eq + cxx * xq;
+    be[qb] = 2.0f * xq + 2.0f * eq;
+    cq[qb] = 4.0f * u * nq[qb];
+    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f;
+  }
+
+  // Upper-bound sets, seeded with the resident row rounded up to FP32.  The
+  // sets hold FP32 bit patterns ordered as integers: upper bounds of
+  // distances are not negative (a negative one sorts below every
+  // non-negative value, which keeps the largest a valid U), -inf pads below
+  // and +inf marks an empty place.
+  const int kg = (k + 3 - grp) / 4;
+  int hu[QB][RK];
+  auto insert = [&](int qb, float xf) __attribute__((always_inline)) {
+    const int x = __float_as_int(xf);
+#pragma unroll
+    for (int i = RK - 1; i >= 1; --i) hu[qb][i] = max(hu[qb][i - 1], min(hu[qb][i], x));
+    hu[qb][0] = min(hu[qb][0], x);
+  };
+  constexpr int kNegInf = (int)0xFF800000u, kPosInf = 0x7F800000;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+    for (int i = 0; i < RK; ++i) hu[qb][i] = i < RK - kg ? kNegInf : kPosInf;
+    if (qv[qb]) {
+      const int j = qbase + ql[qb];
+      const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+      for (int e = 0; e < kg; ++e) insert(qb, __double2float_ru(a.D[row + grp + 4 * e]));
+    }
+  }
+  auto group_u = [&](int qb) -> float {
+    float r = __int_as_float(hu[qb][RK - 1]);
+    r = fmaxf(r, __shfl_xor(r, 16));
+    r = fmaxf(r, __shfl_xor(r, 32));
+    return r;
+  };
+  float uq[QB], thr[QB];
+  auto refresh = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      uq[qb] = group_u(qb);
+      thr[qb] = qv[qb] ? uq[qb] - nq[qb] + dmx[qb] : -__builtin_inff();
+    }
+  };
+  refresh();
+  int lc[QB];  // entries appended to this group's segment of each query's list
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) lc[qb] = 0;
+
+  // Tile staging.  Wave w loads fragment blocks w BPW .. w BPW + BPW - 1 of
+  // each tile (k-step bidx / CB, candidate block bidx % CB): lane l reads
+  // dims 32 s + 8 (l / 16) .. + 8 of candidate (bidx % CB) 16 + l % 16.
+  // Lane l of every wave loads the bound terms of the tile's candidate l
+  // (block l / 16); wave w writes the FP32 norms of block w to LDS.
+  auto cand_pos = [&](int t, int c) { return min(cbeg + t * C + c, cend - 1); };
+  auto bufp = [&](int t) -> unsigned char * { return smem + (size_t)(t & 1) * L.buf; };
+  // Record of candidate `lane` of the next tile, held as loaded until the
+  // tile's loads have landed, and the bound terms of the current tile.
+  float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
+  float rc_x = 0.f, rc_e = 0.f, rc_n = 0.f;
+  int rc_i = 0;
+  auto issue_tile = [&](int t) __attribute__((always_inline)) {
+    unsigned char *b = bufp(t);
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      const int cb = bidx % CB, s = bidx / CB;
+      const __bf16 *src = xg + (size_t)cand_pos(t, cb * 16 + si) * DP + s * 32 + 8 * grp;
+      __builtin_amdgcn_global_load_lds((const void *)src,
+                                       (__attribute__((address_space(3))) void *)(b + L.tile + (size_t)bidx * 1024),
+                                       16, 0, 0);
+    }
+    st = grec[cand_pos(t, lane)];
+  };
+  // After the wave's loads of tile t have landed: the bound terms become
+  // current, and wave w writes the norms of block w.
+  auto finish_tile = [&](int t) __attribute__((always_inline)) {
+    unsigned char *b = bufp(t);
+    const bool ok = cbeg + t * C + lane < cend;
+    rc_n = ok ? st.x : __builtin_inff();
+    rc_x = ok ? st.y : 0.0f;
+    rc_e = ok ? st.z : 0.0f;
+    rc_i = __float_as_int(st.w);
+    if (grp == w) ((float *)(b + L.nc))[lane] = rc_n;
+  };
+  // vmcnt(0), other counters untouched: every load of this wave (the tile's
+  // LDS loads included) has completed.
+  auto wait_loads = []() __attribute__((always_inline)) { __builtin_amdgcn_s_waitcnt(0x0F70); };
+
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
+  if (ntiles > 0) {
+    issue_tile(0);
+    wait_loads();
+    finish_tile(0);
+  }
+
+  for (int t = 0; t < ntiles; ++t) {
+    __syncthreads();  // tile t in LDS; every wave done with tile t - 1
+    const bool more = t + 1 < ntiles;
+    if (more) issue_tile(t + 1);
+    const unsigned char *b = bufp(t);
+    const __bf16 *tA = (const __bf16 *)(b + L.tile);
+    const float *tnc = (const float *)(b + L.nc);
+    f32x4 acc[QB][CB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const bf16x8 av = *(const bf16x8 *)(tA + ((size_t)(s * CB + cb) * 64 + lane) * 8);
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
+      }
+    }
+    // Hot path: one FMA and one min per pair.
+    float tmin[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      tmin[qb] = __builtin_inff();
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = cb * 16 + out_row(lane, v);
+          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
+          acc[qb][cb][v] = tt;
+          tmin[qb] = fminf(tmin[qb], tt);
+        }
+    }
+    bool hit = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) hit |= tmin[qb] <= thr[qb];
+    if (__any(hit)) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        unsigned bits = 0u;
+        if (tmin[qb] <= thr[qb]) {
+#pragma unroll
+          for (int bb = 0; bb < CB * 4; ++bb)
+            bits |= (acc[qb][bb >> 2][bb & 3] <= thr[qb] ? 1u : 0u) << bb;
+        }
+        const int j = qbase + ql[qb];
+        const size_t lb = ((size_t)split * a.n + j) * a.cap + (size_t)grp * capg;
+        // Wave-uniform loop: the record shuffles read other lanes, which
+        // must be active (a shuffle from an inactive lane returns 0).
+        while (__any(bits != 0u)) {
+          const bool act = bits != 0u;
+          const int bb = act ? __builtin_ctz(bits) : 0;
+          bits &= bits - 1;
+          float tt = acc[qb][0][0];
+#pragma unroll
+          for (int b2 = 1; b2 < CB * 4; ++b2)
+            if (bb == b2) tt = acc[qb][b2 >> 2][b2 & 3];
+          const int c = (bb >> 2) * 16 + out_row(lane, bb & 3);
+          // This tile's bound terms (finish_tile of the next tile, after
+          // the survivors, replaces them).
+          const float rx = __shfl(rc_x, c), re = __shfl(rc_e, c), rn = __shfl(rc_n, c);
+          const int rid = __shfl(rc_i, c);
+          if (!act || cbeg + t * C + c >= cend) continue;
+          const float dist = tt + nq[qb];
+          const float delta = (rx * al[qb] + re * be[qb] + 4.0f * u * rn + cq[qb]) *
+                                  1.0009765625f + 1e-30f;
+          const float lower = dist - delta, upper = dist + delta;
+          if (lower <= uq[qb]) {
+            const int at = lc[qb]++;
+            if (at < capg) {
+              a.lst[lb + at] = make_int2(rid, __float_as_int(lower));
+              a.lstu[lb + at] = upper;
+            }
+          }
+          if (__float_as_int(upper) < hu[qb][RK - 1]) insert(qb, upper);
+        }
+      }
+      refresh();
+    }
+    if (more) {
+      wait_loads();
+      finish_tile(t + 1);
+    }
+  }
+
+  float uf[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) uf[qb] = group_u(qb);
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int j = qbase + ql[qb];
+    if (j < a.n) {
+      a.cnt[((size_t)split * 4 + grp) * a.n + j] = lc[qb];
+      if (grp == 0) a.U[(size_t)split * a.n + j] = uf[qb];
+      if (lc[qb] > capg) atomicAdd(a.nover, 1);
     }
   }
 }
@@ -735,12 +1069,15 @@ __device__ void rerank_heap_path(const Bf16Args &a, int j, double *hd, int *hi) 
   const float Ulim = Uf * (1.0f + 0x1p-20f) + 1e-30f;
   double rd = hd[0];
   int ri = hi[0];
+  const int capg = a.cap / a.G;
   for (int sp = 0; sp < a.nsplit; ++sp) {
-    const size_t sj = (size_t)sp * a.n + j;
-    const int cnt = a.cnt[sj];
-    const bool over = cnt > a.cap;
+    bool over = false;
+    for (int gg = 0; gg < a.G; ++gg) over |= list_count(a, sp, gg, j) > capg;
     const int cbeg = sp * a.cps;
-    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : cnt;
+    // An overflowed split is re-scanned once; otherwise each of its lists.
+  for (int gg = 0; gg < (over ? 1 : a.G); ++gg) {
+    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : list_count(a, sp, gg, j);
+    const size_t lb = list_base(a, sp, gg, j);
     for (int base = 0; base < nscan; base += 64) {
       // One list entry (or, for an overflowed list, one candidate) per lane.
       const int e = base + lane;
@@ -751,7 +1088,7 @@ __device__ void rerank_heap_path(const Bf16Args &a, int j, double *hd, int *hi) 
         if (over) {
           cid = a.amap[cbeg + e];
         } else {
-          const int2 en = a.lst[sj * a.cap + e];
+          const int2 en = a.lst[lb + e];
           cid = en.x;
           lw = __int_as_float(en.y);
           keep = lw <= Ulim && (double)lw <= rd;
@@ -791,6 +1128,7 @@ __device__ void rerank_heap_path(const Bf16Args &a, int j, double *hd, int *hi) 
       }
     }
   }
+  }
   if (lane == 0) {
     for (int e2 = k - 1; e2 > 0; --e2) {
       const double xd = hd[e2];
@@ -818,7 +1156,8 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(Bf16Args a, int only_ov
   if (j >= a.n) return;  // whole wave leaves together; no block barrier below
   if (only_over) {
     bool over = false;
-    for (int sp = 0; sp < a.nsplit; ++sp) over |= a.cnt[(size_t)sp * a.n + j] > a.cap;
+    for (int sp = 0; sp < a.nsplit; ++sp)
+      for (int gg = 0; gg < a.G; ++gg) over |= list_count(a, sp, gg, j) > a.cap / a.G;
     if (!over) return;
   }
   rerank_heap_path(a, j, hd, hi);
@@ -889,10 +1228,11 @@ __global__ __launch_bounds__(256) void knn_rerank_select_kernel(Bf16Args a) {
   int *pi = (int *)(wbase + SL.pi);
   if (j >= a.n) return;  // whole wave leaves together; no block barrier below
   bool over = false;
-  int ntot = k;  // keys: the resident row, then each split's list
-  for (int sp = 0; sp < a.nsplit; ++sp) {
-    const int c = a.cnt[(size_t)sp * a.n + j];
-    over |= c > a.cap;
+  const int nl = a.nsplit * a.G;  // lists of this query: list L is (L / G, L % G)
+  int ntot = k;  // keys: the resident row, then each list's entries
+  for (int L = 0; L < nl; ++L) {
+    const int c = list_count(a, L / a.G, L % a.G, j);
+    over |= c > a.cap / a.G;
     ntot += c;
   }
   if (over) return;
@@ -918,11 +1258,10 @@ __global__ __launch_bounds__(256) void knn_rerank_select_kernel(Bf16Args a) {
     atomicAdd(&hist[key >> 24], 1u);
   }
   {
-    int kbase = k;  // key index of this split's first entry
-    for (int sp = 0; sp < a.nsplit; ++sp) {
-      const size_t sj = (size_t)sp * a.n + j;
-      const int cnt = a.cnt[sj];
-      const float *up = a.lstu + sj * a.cap;
+    int kbase = k;  // key index of this list's first entry
+    for (int L = 0; L < nl; ++L) {
+      const int cnt = list_count(a, L / a.G, L % a.G, j);
+      const float *up = a.lstu + list_base(a, L / a.G, L % a.G, j);
       for (int base = 0; base < cnt; base += 64 * kSelUnroll) {
         float v[kSelUnroll];
 #pragma unroll
@@ -954,15 +1293,14 @@ __global__ __launch_bounds__(256) void knn_rerank_select_kernel(Bf16Args a) {
       auto count = [&](unsigned key) __attribute__((always_inline)) {
         if (((key >> sh) >> 8) == prefix) atomicAdd(&hist[(key >> sh) & 255u], 1u);
       };
-      const int nl = ntot < kSelKeys ? ntot : kSelKeys;
-      for (int e = lane; e < nl; e += 64) count(keys[e]);
+      const int ns = ntot < kSelKeys ? ntot : kSelKeys;
+      for (int e = lane; e < ns; e += 64) count(keys[e]);
       if (ntot > kSelKeys) {
         // Keys beyond the LDS stage are read again from the lists.
         int kbase = k;
-        for (int sp = 0; sp < a.nsplit; ++sp) {
-          const size_t sj = (size_t)sp * a.n + j;
-          const int cnt = a.cnt[sj];
-          const float *up = a.lstu + sj * a.cap;
+        for (int L = 0; L < nl; ++L) {
+          const int cnt = list_count(a, L / a.G, L % a.G, j);
+          const float *up = a.lstu + list_base(a, L / a.G, L % a.G, j);
           for (int e = lane; e < cnt; e += 64)
             if (kbase + e >= kSelKeys) count(fkey(up[e]));
           kbase += cnt;
@@ -1074,10 +1412,9 @@ __global__ __launch_bounds__(256) void knn_rerank_select_kernel(Bf16Args a) {
   // 2a. Compact the entries at or below T into the pool.
   // A batch whose survivors would overflow the pool is taken one window at a
   // time, cutting the pool back to k before a window that does not fit.
-  for (int sp = 0; sp < a.nsplit; ++sp) {
-    const size_t sj = (size_t)sp * a.n + j;
-    const int cnt = a.cnt[sj];
-    const int2 *ls = a.lst + sj * a.cap;
+  for (int L = 0; L < nl; ++L) {
+    const int cnt = list_count(a, L / a.G, L % a.G, j);
+    const int2 *ls = a.lst + list_base(a, L / a.G, L % a.G, j);
     for (int base = 0; base < cnt; base += 64 * kSelUnroll) {
       int2 en[kSelUnroll];
 #pragma unroll
@@ -1137,9 +1474,13 @@ namespace {
 constexpr size_t kLds = 160 * 1024;
 constexpr int kCBb = 4;
 
+constexpr int kRegRK = 16;  // register form: upper-bound set size per lane (k <= 4 kRegRK)
+
 struct Bf16Plan {
   int ks = 0, qb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
-  bool pl = false;  // per-lane upper-bound heaps
+  bool pl = false;   // per-lane upper-bound heaps
+  bool reg = false;  // register form of the filter (knn_bf16_filter2_kernel)
+  int G = 1;         // list segments per (split, query)
   size_t lds = 0;
 };
 
@@ -1201,24 +1542,46 @@ Bf16Plan plan_bf16(int d, int k) {
   const int cap = (p.pl ? 32 : 16) * k;
   const int lo = p.pl ? 1024 : 512, hi = p.pl ? 8192 : 4096;
   p.cap = cap < lo ? lo : (cap > hi ? hi : cap);
+  // Register form for 48 <= k <= 4 kRegRK (two queries blocks, four waves;
+  // GSKNN_BF16_REG=0 keeps the LDS-heap form, for comparison).
+  static const bool reg_on = [] {
+    const char *e = getenv("GSKNN_BF16_REG");
+    return !(e && atoi(e) == 0);
+  }();
+  if (reg_on && p.pl && k <= 4 * kRegRK && Filter2Layout(p.ks, kCBb).bytes <= kLds) {
+    p.reg = true;
+    p.qb = 2;
+    p.w = 4;
+    p.q = 4 * 16 * 2;
+    p.lds = Filter2Layout(p.ks, kCBb).bytes;
+    p.G = 4;
+  }
   return p;
 }
 
-size_t bf16_work_layout(int n, int cap, int nsplit, size_t *o_lst, size_t *o_cnt, size_t *o_u,
-                        size_t *o_over, size_t *o_lstu = nullptr) {
+// panel_bytes: the register form's candidate panel (0 for the other forms).
+size_t bf16_work_layout(int n, int cap, int nsplit, int G, size_t panel_bytes, size_t *o_lst,
+                        size_t *o_cnt, size_t *o_u, size_t *o_over, size_t *o_lstu = nullptr,
+                        size_t *o_panel = nullptr) {
   size_t off = 0;
   const size_t rows = (size_t)n * nsplit;
   *o_lst = off;
   off = al16r(off + rows * cap * sizeof(int2));
   *o_cnt = off;
-  off = al16r(off + rows * 4);
+  off = al16r(off + rows * G * 4);
   *o_u = off;
   off = al16r(off + rows * 4);
   *o_over = off;
   off = al16r(off + 16);
   if (o_lstu) *o_lstu = off;
   off = al16r(off + rows * cap * sizeof(float));
+  if (o_panel) *o_panel = off;
+  off = al16r(off + panel_bytes);
   return off;
+}
+
+size_t panel_bytes(const Bf16Plan &p, int m, int d) {
+  return p.reg ? PanelLayout(m, bf16_row_dims(d)).bytes : 0;
 }
 
 using SelKernel = void (*)(Bf16Args);
@@ -1235,6 +1598,14 @@ SelKernel sel_kernel(int ns) {
 template <int KS>
 hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s) {
   const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
+  if (p.reg) {
+    auto kern = knn_bf16_filter2_kernel<KS, 2, kCBb, 4, kRegRK>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)p.lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(256), p.lds, s, a);
+    return hipGetLastError();
+  }
 #define GSKNN_BF16_CASE(QB_, W_)                                                       \
   if (p.qb == QB_ && p.w == W_) {                                                      \
     auto kern = p.pl ? knn_bf16_filter_kernel<KS, QB_, kCBb, W_, true>                  \
@@ -1261,7 +1632,7 @@ size_t bf16_work_bytes(int m, int n, int d, int k) {
   if (p.ks == 0 || p.q == 0 || n <= 0 || m <= 0) return 0;
   plan_splits(p, m, n);
   size_t a, b, c, e;
-  return bf16_work_layout(n, p.cap, p.nsplit, &a, &b, &c, &e);
+  return bf16_work_layout(n, p.cap, p.nsplit, p.G, panel_bytes(p, m, d), &a, &b, &c, &e);
 }
 
 size_t bf16_count_offset(int m, int n, int d, int k, int *nsplit) {
@@ -1269,8 +1640,8 @@ size_t bf16_count_offset(int m, int n, int d, int k, int *nsplit) {
   if (p.ks == 0 || p.q == 0 || n <= 0 || m <= 0) return 0;
   plan_splits(p, m, n);
   size_t o_lst, o_cnt, o_u, o_over;
-  bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over);
-  if (nsplit) *nsplit = p.nsplit;
+  bf16_work_layout(n, p.cap, p.nsplit, p.G, panel_bytes(p, m, d), &o_lst, &o_cnt, &o_u, &o_over);
+  if (nsplit) *nsplit = p.nsplit * p.G;  // lists per query
   return o_cnt;
 }
 
@@ -1285,11 +1656,13 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   const size_t rl = al16r(sizeof(double) * 4 * (size_t)k) + sizeof(int) * 4 * (size_t)k;
   if (rl > kLds) return hipErrorNotSupported;
   size_t o_lst, o_cnt, o_u, o_over, o_lstu;
-  const size_t need = bf16_work_layout(n, p.cap, p.nsplit, &o_lst, &o_cnt, &o_u, &o_over, &o_lstu);
+  size_t o_panel;
+  const size_t need = bf16_work_layout(n, p.cap, p.nsplit, p.G, panel_bytes(p, m, d), &o_lst, &o_cnt,
+                                       &o_u, &o_over, &o_lstu, &o_panel);
   if (work == nullptr || work_bytes < need) return hipErrorInvalidValue;
   Bf16Args a{};
   a.m = m; a.n = n; a.d = d; a.k = k; a.ldk = ldk; a.cap = p.cap;
-  a.nsplit = p.nsplit; a.cps = p.cps;
+  a.nsplit = p.nsplit; a.cps = p.cps; a.G = p.G;
   a.NA = NA; a.NB = NB;
   a.XA = XA; a.XB = XB;
   a.prepA = (const unsigned char *)prepA;
@@ -1298,10 +1671,14 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   a.D = D; a.I = I; a.hrow = hrow;
   a.lst = (int2 *)((char *)work + o_lst);
   a.lstu = (float *)((char *)work + o_lstu);
+  a.panel = (const unsigned char *)work + o_panel;
   a.cnt = (int *)((char *)work + o_cnt);
   a.U = (float *)((char *)work + o_u);
   a.nover = (int *)((char *)work + o_over);
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, a.nover);
+  if (p.reg)
+    hipLaunchKernelGGL(bf16_gather_kernel, dim3((m + 3) / 4), dim3(256), 0, s, m, NA, d, amap,
+                       a.prepA, (unsigned char *)work + o_panel);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   switch (p.ks) {
