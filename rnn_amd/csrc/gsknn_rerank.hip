@@ -638,19 +638,23 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
 //    bound terms and id of the tile's candidate l in registers and survivors
 //    fetch them by lane shuffles (no LDS reads after the next tile's loads
 //    are issued, which the compiler would make wait for those loads).
-// Candidate panel of the register form: BF16 rows [m][DP] in candidate
-// position order, then a record {nc, xn, en, id} per position (nc = the FP64
-// squared norm rounded to FP32, id = amap[position]).
+// Candidate panel of the register form: BF16 rows [m + kPanelPad][DP] in
+// candidate position order, then a record {nc, xn, en, id} per position
+// (nc = the FP64 squared norm rounded to FP32, id = amap[position]).  The
+// kPanelPad zero rows and records after the last position let a tile that
+// runs past the end load without clamping (its extra candidates are masked).
+constexpr int kPanelPad = 64;
 struct PanelLayout {
   size_t rows, rec, bytes;
   __host__ __device__ PanelLayout(int m, int DP) {
     rows = 0;
-    rec = al16r((size_t)m * DP * 2);
-    bytes = al16r(rec + (size_t)m * 16);
+    rec = al16r((size_t)(m + kPanelPad) * DP * 2);
+    bytes = al16r(rec + (size_t)(m + kPanelPad) * 16);
   }
 };
 
-// One wave per candidate position: 16 bytes of the row per lane.
+// One wave per candidate position (the pad positions written as zeros):
+// 16 bytes of the row per lane.
 __global__ __launch_bounds__(256) void bf16_gather_kernel(int m, int NA, int d, const int *amap,
                                                           const unsigned char *prepA,
                                                           unsigned char *panel) {
@@ -659,7 +663,13 @@ __global__ __launch_bounds__(256) void bf16_gather_kernel(int m, int NA, int d, 
   const PanelLayout P(m, DP);
   const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (pos >= m) return;
+  if (pos >= m + kPanelPad) return;
+  if (pos >= m) {
+    uint4 *dst = (uint4 *)(panel + P.rows + (size_t)pos * DP * 2);
+    for (int c = lane; c < DP / 8; c += 64) dst[c] = make_uint4(0u, 0u, 0u, 0u);
+    if (lane == 0) ((float4 *)(panel + P.rec))[pos] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   const int id = amap[pos];
   const uint4 *src = (const uint4 *)(prepA + PA.xb + (size_t)id * DP * 2);
   uint4 *dst = (uint4 *)(panel + P.rows + (size_t)pos * DP * 2);
@@ -789,7 +799,17 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   // dims 32 s + 8 (l / 16) .. + 8 of candidate (bidx % CB) 16 + l % 16.
   // Lane l of every wave loads the bound terms of the tile's candidate l
   // (block l / 16); wave w writes the FP32 norms of block w to LDS.
-  auto cand_pos = [&](int t, int c) { return min(cbeg + t * C + c, cend - 1); };
+  // Tile t covers positions cbeg + t C .. + C - 1 (the panel's pad rows
+  // cover a last tile that runs past the end): a uniform base per tile and
+  // fixed per-lane offsets.
+  auto tile_rows = [&](int t) { return (const char *)xg + (size_t)(cbeg + t * C) * DP * 2; };
+  auto tile_recs = [&](int t) { return grec + cbeg + t * C; };
+  unsigned doff[BPW];  // byte offset of this lane's 16 bytes of block w BPW + e
+#pragma unroll
+  for (int e = 0; e < BPW; ++e) {
+    const int bidx = w * BPW + e;
+    doff[e] = (unsigned)(((bidx % CB) * 16 + si) * DP * 2 + ((bidx / CB) * 32 + 8 * grp) * 2);
+  }
   auto bufp = [&](int t) -> unsigned char * { return smem + (size_t)(t & 1) * L.buf; };
   // Record of candidate `lane` of the next tile, held as loaded until the
   // tile's loads have landed, and the bound terms of the current tile.
@@ -798,16 +818,15 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   int rc_i = 0;
   auto issue_tile = [&](int t) __attribute__((always_inline)) {
     unsigned char *b = bufp(t);
+    const char *rows = tile_rows(t);
 #pragma unroll
     for (int e = 0; e < BPW; ++e) {
       const int bidx = w * BPW + e;
-      const int cb = bidx % CB, s = bidx / CB;
-      const __bf16 *src = xg + (size_t)cand_pos(t, cb * 16 + si) * DP + s * 32 + 8 * grp;
-      __builtin_amdgcn_global_load_lds((const void *)src,
+      __builtin_amdgcn_global_load_lds((const void *)(rows + doff[e]),
                                        (__attribute__((address_space(3))) void *)(b + L.tile + (size_t)bidx * 1024),
                                        16, 0, 0);
     }
-    st = grec[cand_pos(t, lane)];
+    st = tile_recs(t)[lane];
   };
   // After the wave's loads of tile t have landed: the bound terms become
   // current, and wave w writes the norms of block w.
@@ -1677,7 +1696,7 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   a.nover = (int *)((char *)work + o_over);
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, a.nover);
   if (p.reg)
-    hipLaunchKernelGGL(bf16_gather_kernel, dim3((m + 3) / 4), dim3(256), 0, s, m, NA, d, amap,
+    hipLaunchKernelGGL(bf16_gather_kernel, dim3((m + kPanelPad + 3) / 4), dim3(256), 0, s, m, NA, d, amap,
                        a.prepA, (unsigned char *)work + o_panel);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
