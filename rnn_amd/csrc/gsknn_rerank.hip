@@ -25,16 +25,19 @@
 // (times 1 + 2^-10 for the rounding of delta itself).  lower = dist~ - delta
 // and upper = dist~ + delta bracket D.
 //
-// Pass 1 (knn_bf16_filter_kernel): per query, U = k-th smallest of
-// {upper bounds of all candidates} U {resident heap distances}; U bounds the
-// final k-th distance from above.  Every candidate whose lower bound is <= U
-// at the time it is seen is appended to the query's list (id, lower, upper).
-// Pass 2 (knn_rerank_kernel): one wave per query reads its list 64 entries
-// at a time, drops entries with lower > final U or above the row's current
-// k-th distance (which falls as entries are merged), computes sum_p (a_p - b_p)^2
-// in FP64 for the rest (four at a time, 16 lanes each, fixed summation order)
-// and merges them into the resident row under the canonical order.  A query
-// whose list overflowed is re-scanned against every candidate in FP64.
+// Pass 1 (knn_bf16_filter2_kernel for 48 <= k <= 64, knn_bf16_filter_kernel
+// otherwise): per query, U = k-th smallest of {upper bounds of all
+// candidates} U {resident heap distances} (or a valid bound above it); U
+// bounds the final k-th distance from above.  Every candidate whose lower
+// bound is <= U at the time it is seen is appended to the query's list (id,
+// lower, upper).
+// Pass 2 (knn_rerank_select_kernel for k <= 256): one wave per query takes T,
+// the k-th smallest of the listed upper bounds and the resident row, computes
+// FP64 direct differences sum_p (a_p - b_p)^2 (wave_row_dists: a fixed
+// summation order per pair) for the entries whose lower bound is <= T, and
+// keeps the k smallest of (resident row + those) in canonical order.  A query
+// whose list overflowed, and every query for larger k, goes to the heap form
+// (knn_rerank_kernel), which re-scans an overflowed split in FP64.
 #include <limits.h>
 
 #include "gsknn_common.h"
