@@ -292,10 +292,12 @@ def allnn_sweep(torch, dev, args):
     """All-NN tree-sweep workload (`--workload c4`): N = 2^20 points, d = 32, FP32, 256-cluster mixture; 8
     randomized-projection-tree iterations with leaves of 256-4096 points, k = 32,
     global heaps carried across iterations.  One persistent launch per
-    iteration (graph-replayed after the first), autotuned launch shape; the
-    next iteration's tree is built on a second stream during the sweep.
-    tree_ms and sweep_ms are device times of each part, iteration_ms the
-    wall time of one step with the next tree overlapped."""
+    iteration: the exact VALU form on the empty rows of iteration 0, the FP32
+    MFMA form after (direct once, then graph-replayed), autotuned launch
+    shapes; trees are built four at a time on a second stream during the
+    sweeps.  tree_ms is the device time of a batch's build divided by its
+    trees, sweep_ms the sweep's device time, iteration_ms the wall time of one
+    step (the first step of each form includes its autotuning)."""
     from rnn_amd import datasets as ds
     from rnn_amd import sweep
 
@@ -304,24 +306,33 @@ def allnn_sweep(torch, dev, args):
     Xn = ds.config4_points(N, 32, 256, seed=3)
     log(f"c4 data N={N} d=32 ({time.perf_counter() - t0:.1f}s)")
     X = torch.from_numpy(Xn).to(dev)
-    sw = sweep.RPTreeSweep(X, 32, seed=3, leaf_lo=256, leaf_hi=4096, log=log)
+    sw = sweep.RPTreeSweep(X, 32, seed=3, leaf_lo=256, leaf_hi=4096, log=log,
+                           iterations=args.iters)
     for _ in range(args.iters):
         _, _, info = sw.step()
-        log(f"iteration {info['iteration']}: tree {info['tree_ms']:.2f} ms, sweep "
-            f"{info['sweep_ms']:.3f} ms, iteration {info['iteration_ms']:.2f} ms, "
+        log(f"iteration {info['iteration']} ({info['form']}): tree {info['tree_ms']:.2f} ms, "
+            f"sweep {info['sweep_ms']:.3f} ms, iteration {info['iteration_ms']:.2f} ms, "
             f"{info['leaves']} leaves {info['leaf_sizes']}")
+    sw.close()
     t = sw.timings
     pairs = sum(i["pairs"] for i in t)
     sweep_ms = sum(i["sweep_ms"] for i in t)
+    tuned = {}
+    for name, cfg, info in (("exact", sw.config, sw.tune_info),
+                            ("mfma", sw.config_mfma, sw.tune_info_mfma)):
+        if cfg is not None:
+            tuned[name] = {"qpb": cfg[0], "ct": cfg[1], "grid": cfg[2],
+                           "tried": (info or {}).get("tried")}
     out = {"workload": "c4", "N": N, "d": 32, "k": 32, "iterations": len(t),
+           "form_per_iteration": [i["form"] for i in t],
            "sweep_ms_per_iteration": [i["sweep_ms"] for i in t],
            "tree_ms_per_iteration": [i["tree_ms"] for i in t],
+           "tree_batch": sw.batch,
            "iteration_ms": [i["iteration_ms"] for i in t],
            "pairs_per_iteration": [i["pairs"] for i in t],
            "pairs_per_s": pairs / (sweep_ms * 1e-3),
-           "graph_replayed_iterations": len(t) - 1,
-           "autotuned": {"qpb": sw.config[0], "ct": sw.config[1], "grid": sw.config[2],
-                         "tried": (sw.tune_info or {}).get("tried")}}
+           "graph_replayed_iterations": sum(1 for i in t if i["graph"]),
+           "autotuned": tuned}
     return out
 
 
@@ -390,7 +401,7 @@ def north_star_objects(torch, dev, args):
                                     "rows_equal_bf16_vs_fp64_gpu", "sample_rows",
                                     "sample_bruteforce_ok_bf16",
                                     "bf16_max_rel_dist_err_vs_fp64_bruteforce")),
-            ("c4", allnn_sweep, ("N", "iterations", "tree_ms_per_iteration",
+            ("c4", allnn_sweep, ("N", "iterations", "form_per_iteration", "tree_ms_per_iteration",
                                  "sweep_ms_per_iteration", "iteration_ms", "pairs_per_s",
                                  "graph_replayed_iterations")),
             ("c5", serving, ("requests", "queries", "rate_req_per_s", "offered_pairs_per_s",

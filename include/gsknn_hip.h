@@ -165,6 +165,22 @@ int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
                             const int *nitems, double *D, int *I, int ldk,
                             int qpb, int ct, int grid, void *stream);
 
+/* FP32 MFMA form of the same sweep (no counterpart in the reference): the
+ * same work list, heap rows and results (distances in the reference oracle's
+ * arithmetic, bit for bit); v_mfma_f32_16x16x4f32 only filters, under a
+ * rigorous error bound, which pairs get their oracle distance formed.
+ * gsknn_sweep_norms_sdev computes once per table xn[i] = |X[i]| (rounded
+ * up) and mx = {max xn, max |X2|}, which the bound needs.  Supported shapes:
+ * gsknn_sweep_mfma_supported(d, k, qpb) (d in {16, 32}, k <= 64, qpb = 32);
+ * grid = persistent workgroups of one wave. */
+int gsknn_sweep_norms_sdev(int N, int d, const float *X, const float *X2, float *xn,
+                           float *mx, void *stream);
+int gsknn_sweep_mfma_supported(int d, int k, int qpb);
+int gsknn_sweep_leaves_mfma_sdev(int d, int k, const float *X, const float *X2,
+                                 const float *xn, const float *mx, const int *perm,
+                                 const int *items, const int *nitems, float *D,
+                                 int *I, int ldk, int qpb, int grid, void *stream);
+
 /* Sorted output: the n heap rows (row j at D + j*ldk, I + j*ldk, k <= 4096)
  * rewritten in place in ascending (distance, index) order, e.g. after a
  * gsknn call.  No entry point of the reference library does this; its tree

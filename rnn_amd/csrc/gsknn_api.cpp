@@ -350,6 +350,34 @@ extern "C" int gsknn_sweep_leaves_sdev(int d, int k, const float *X, const float
                    nitems, D, I, ldk, qpb, ct, grid, stream);
 }
 
+extern "C" int gsknn_sweep_norms_sdev(int N, int d, const float *X, const float *X2, float *xn,
+                                      float *mx, void *stream) {
+  const char *fn = "gsknn_sweep_norms_sdev";
+  if (N < 0 || d <= 0) return set_err(GSKNN_EINVAL, "%s: N=%d d=%d", fn, N, d);
+  if (N > 0 && (!X || !X2 || !xn || !mx)) return set_err(GSKNN_EINVAL, "%s: null array", fn);
+  hipError_t e = launch_sweep_norms(N, d, X, X2, xn, mx, (hipStream_t)stream);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, fn);
+}
+
+extern "C" int gsknn_sweep_mfma_supported(int d, int k, int qpb) {
+  return leaf_sweep_mfma_ok(d, k, qpb) ? 1 : 0;
+}
+
+extern "C" int gsknn_sweep_leaves_mfma_sdev(int d, int k, const float *X, const float *X2,
+                                            const float *xn, const float *mx, const int *perm,
+                                            const int *items, const int *nitems, float *D, int *I,
+                                            int ldk, int qpb, int grid, void *stream) {
+  const char *fn = "gsknn_sweep_leaves_mfma_sdev";
+  if (d <= 0 || k <= 0 || ldk < k) return set_err(GSKNN_EINVAL, "%s: d=%d k=%d ldk=%d", fn, d, k, ldk);
+  if (!X || !X2 || !xn || !mx || !perm || !items || !nitems || !D || !I)
+    return set_err(GSKNN_EINVAL, "%s: null array", fn);
+  hipError_t e = launch_leaf_sweep_mfma(d, k, X, X2, xn, mx, perm, items, nitems, D, I, ldk, qpb,
+                                        grid, (hipStream_t)stream);
+  if (e == hipErrorNotSupported)
+    return set_err(GSKNN_ELIMIT, "%s: d=%d k=%d qpb=%d grid=%d not supported", fn, d, k, qpb, grid);
+  return e == hipSuccess ? GSKNN_OK : hip_err(e, fn);
+}
+
 extern "C" int gsknn_sweep_leaves_ddev(int d, int k, const double *X, const double *X2,
                                        const int *perm, const int *items, const int *nitems,
                                        double *D, int *I, int ldk, int qpb, int ct, int grid,

@@ -128,6 +128,15 @@ hipError_t launch_leaf_sweep_f64(int d, int k, const double *X, const double *X2
                                  const int *items, const int *nitems, double *D, int *I, int ldk,
                                  int qpb, int ct, int grid, hipStream_t s);
 size_t leaf_sweep_lds_bytes(int elem, int d, int k, int qpb, int ct);
+// FP32 MFMA form of the leaf sweep (gsknn_sweep.hip): point norms, support
+// test and launch.
+hipError_t launch_sweep_norms(int N, int d, const float *X, const float *X2, float *xn, float *mx,
+                              hipStream_t s);
+bool leaf_sweep_mfma_ok(int d, int k, int qpb);
+hipError_t launch_leaf_sweep_mfma(int d, int k, const float *X, const float *X2, const float *xn,
+                                  const float *mx, const int *perm, const int *items,
+                                  const int *nitems, float *D, int *I, int ldk, int qpb, int grid,
+                                  hipStream_t s);
 // One tree level's positions and sort keys (gsknn_sweep.hip).
 hipError_t launch_tree_level_keys(long long total, int nnodes, int d, int id_bits,
                                   const float *X, const long long *perm,

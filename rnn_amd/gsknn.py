@@ -94,6 +94,9 @@ def _declare(L):
     L.gsknn_dsqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_ssqnorm_dev.argtypes = [i32, i32, vp, vp, vp]
     L.gsknn_tree_level_keys_dev.argtypes = [ctypes.c_longlong, i32, i32, i32] + [vp] * 8
+    L.gsknn_sweep_norms_sdev.argtypes = [i32, i32] + [vp] * 5
+    L.gsknn_sweep_mfma_supported.argtypes = [i32] * 3
+    L.gsknn_sweep_leaves_mfma_sdev.argtypes = [i32] * 2 + [vp] * 9 + [i32] * 3 + [vp]
     L.gsknn_rows_ascending_ddev.argtypes = [i32, i32, vp, vp, i32, vp]
     L.gsknn_rows_ascending_sdev.argtypes = [i32, i32, vp, vp, i32, vp]
     for name in ("dgsknn_var1", "sgsknn_var1"):
@@ -323,6 +326,23 @@ def sqnorm_dev(X, X2, stream=None):
     rc = fn(X.shape[0], X.shape[1], _dp(X), _dp(X2), _stream_ptr(stream))
     if rc:
         raise GsknnError(last_error())
+
+
+def sweep_norms_dev(X, X2, stream=None):
+    """Norms the FP32 MFMA leaf sweep's error bound needs: xn (N,) with
+    xn[i] = |X[i]| rounded up, and mx (2,) = {max xn, max |X2|}."""
+    import torch
+    xn = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+    mx = torch.zeros(2, dtype=torch.float32, device=X.device)
+    rc = lib().gsknn_sweep_norms_sdev(X.shape[0], X.shape[1], _dp(X), _dp(X2), _dp(xn), _dp(mx),
+                                      _stream_ptr(stream))
+    if rc:
+        raise GsknnError(last_error())
+    return xn, mx
+
+
+def sweep_mfma_supported(d, k, qpb=32) -> bool:
+    return bool(lib().gsknn_sweep_mfma_supported(d, k, qpb))
 
 
 def tree_level_keys_dev(X, perm, starts, offs, dirs, total, id_bits, pos, keys, stream=None):

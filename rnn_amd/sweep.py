@@ -14,7 +14,9 @@ On the device each tree level is one fused kernel of libgsknn.so
 point) and one sort of the keys.
 
 The leaves of one iteration run in ONE persistent launch of the leaf-sweep
-kernel (libgsknn.so: gsknn_sweep_leaves_[sd]dev) over a device work list;
+kernel (libgsknn.so: gsknn_sweep_leaves_mfma_sdev, the FP32 MFMA form, for
+d in {16, 32} and k <= 64; gsknn_sweep_leaves_[sd]dev, the exact VALU form,
+otherwise) over a device work list;
 that launch is captured once in a hipGraph and replayed for every iteration
 after the tree build rewrites the work list in place.  The launch shape
 (queries per workgroup, candidates per LDS tile, persistent grid) comes from
@@ -86,47 +88,94 @@ def build_tree(X, iteration, seed, leaf_lo=256, leaf_hi=4096, perm_out=None, pla
     Returns (perm, leaves): perm an int32 device tensor of point ids grouped
     by leaf (written into perm_out when given), leaves a list of (start,
     size) in perm order.  The node structure comes from plan_tree on the
-    host (or `plan`, its result); the per-level node arrays go to the device
-    in one copy, and each level is one fused key kernel (libgsknn.so) and one
-    sort of the keys, on the current stream with no host synchronisation.
+    host (or `plan`, its result).  See build_trees.
+    """
+    outs = None if perm_out is None else [perm_out]
+    perms, leaves = build_trees(X, [iteration], seed, leaf_lo, leaf_hi, perm_outs=outs,
+                                plans=None if plan is None else [plan])
+    return perms[0], leaves[0]
+
+
+def build_trees(X, iterations, seed, leaf_lo=256, leaf_hi=4096, perm_outs=None, plans=None):
+    """The trees of several iterations at once (rows of X, (N, d) float32).
+
+    Returns (perms, leaves): one int32 tensor of point ids grouped by leaf per
+    tree (written into perm_outs[t] when given) and one leaf list of (start,
+    size) per tree.  On the device the trees advance together level by
+    level: each level is one fused key kernel over the split nodes of every
+    tree (libgsknn.so: node lookup, projection and packed sort key per point)
+    and one sort of the keys, the key's node field numbering the nodes of all
+    trees in order; a level whose node count does not fit the 63-bit key
+    beside the projection and the point id is sorted tree by tree.  The per-
+    level node arrays go to the device in one copy; no host synchronisation.
     """
     import torch
     dev = X.device
     N, d = X.shape
-    levels, leaves = plan if plan is not None else plan_tree(N, d, iteration, seed, leaf_lo, leaf_hi)
-    perm = torch.arange(N, device=dev, dtype=torch.int64)
+    T = len(iterations)
+    if plans is None:
+        plans = [plan_tree(N, d, it, seed, leaf_lo, leaf_hi) for it in iterations]
     id_bits = max(1, int(N - 1).bit_length())
-    fused = X.is_cuda and all(max(1, int(st.size - 1).bit_length()) + 32 + id_bits <= 63
-                              for st, _, _ in levels)
-    if levels and fused:
-        # One upload: per level, starts and element offsets (int64), then
-        # the directions (float32).
-        nodes = np.array([st.size for st, _, _ in levels], dtype=np.int64)
-        node_off = np.concatenate([[0], np.cumsum(nodes)])
-        ints = np.concatenate([np.concatenate([st, np.cumsum(sz) - sz]) for st, sz, _ in levels])
-        dirs = np.concatenate([dr for _, _, dr in levels]).astype(np.float32)
-        ints_t = torch.from_numpy(ints).pin_memory().to(dev, non_blocking=True)
-        dirs_t = torch.from_numpy(dirs).pin_memory().to(dev, non_blocking=True)
-        int_off = 0
-        for li, (starts, sizes, _) in enumerate(levels):
-            nn = starts.size
-            starts_t = ints_t[int_off:int_off + nn]
-            offs_t = ints_t[int_off + nn:int_off + 2 * nn]
-            int_off += 2 * nn
-            total = int(sizes.sum())
+    perm = torch.arange(N, device=dev, dtype=torch.int64).repeat(T)  # tree t at t N
+    nlev = max((len(pl[0]) for pl in plans), default=0)
+    if X.is_cuda and nlev:
+        # Per level and group of trees sorted together: node starts (with
+        # the tree's offset), element offsets and directions.
+        groups = []  # (level, [trees], fused)
+        for li in range(nlev):
+            trees = [t for t in range(T) if li < len(plans[t][0])]
+            nodes = sum(plans[t][0][li][0].size for t in trees)
+            if max(1, int(nodes - 1).bit_length()) + 32 + id_bits <= 63:
+                groups.append((li, trees, True))
+                continue
+            for t in trees:
+                nodes = plans[t][0][li][0].size
+                groups.append((li, [t], max(1, int(nodes - 1).bit_length()) + 32 + id_bits <= 63))
+        ints, dirs, meta = [np.zeros(0, np.int64)], [np.zeros(0, np.float32)], []
+        ioff = doff = 0
+        for li, trees, fused in groups:
+            if not fused:
+                meta.append(None)
+                continue
+            st = np.concatenate([plans[t][0][li][0] + t * N for t in trees])
+            sz = np.concatenate([plans[t][0][li][1] for t in trees])
+            dr = np.concatenate([plans[t][0][li][2] for t in trees]).astype(np.float32)
+            ints.append(np.concatenate([st, np.cumsum(sz) - sz]))
+            dirs.append(dr.reshape(-1))
+            meta.append((st.size, int(sz.sum()), ioff, doff))
+            ioff += 2 * st.size
+            doff += dr.size
+        ints_t = torch.from_numpy(np.concatenate(ints)).pin_memory().to(dev, non_blocking=True)
+        dirs_t = torch.from_numpy(np.concatenate(dirs)).pin_memory().to(dev, non_blocking=True)
+        for (li, trees, fused), m in zip(groups, meta):
+            if not fused:
+                # Too many nodes for the packed key: this tree's level with
+                # torch operations (two sorts).
+                t = trees[0]
+                _split_level_torch(X, perm[t * N:(t + 1) * N], *plans[t][0][li], id_bits)
+                continue
+            nn, total, io, do = m
+            starts_t = ints_t[io:io + nn]
+            offs_t = ints_t[io + nn:io + 2 * nn]
             pos = torch.empty(total, dtype=torch.int64, device=dev)
             key = torch.empty(total, dtype=torch.int64, device=dev)
-            gsknn.tree_level_keys_dev(X, perm, starts_t, offs_t,
-                                      dirs_t[node_off[li]:node_off[li + 1]], total, id_bits,
-                                      pos, key)
+            gsknn.tree_level_keys_dev(X, perm, starts_t, offs_t, dirs_t[do:do + nn * d].view(nn, d),
+                                      total, id_bits, pos, key)
             perm[pos] = torch.sort(key).values & ((1 << id_bits) - 1)
     else:
-        for starts, sizes, dirs in levels:
-            _split_level_torch(X, perm, starts, sizes, dirs, id_bits)
-    if perm_out is not None:
-        perm_out.copy_(perm)
-        return perm_out, leaves
-    return perm.to(torch.int32), leaves
+        for t in range(T):
+            pt = perm[t * N:(t + 1) * N]
+            for starts, sizes, dirs_ in plans[t][0]:
+                _split_level_torch(X, pt, starts, sizes, dirs_, id_bits)
+    perms = []
+    for t in range(T):
+        pt = perm[t * N:(t + 1) * N]
+        if perm_outs is not None:
+            perm_outs[t].copy_(pt)
+            perms.append(perm_outs[t])
+        else:
+            perms.append(pt.to(torch.int32))
+    return perms, [pl[1] for pl in plans]
 
 
 def _split_level_torch(X, perm, starts, sizes, dirs, id_bits):
@@ -160,13 +209,16 @@ def make_items(leaves, qpb):
     """Work list [nitems][4] = (start, size, first query, 0), largest leaves
     first (ties in leaf order, query blocks ascending)."""
     lv = np.asarray(leaves, dtype=np.int64).reshape(-1, 2)
+    # Ordering the leaves (stable) and then expanding them orders the items.
+    lv = lv[np.argsort(-lv[:, 1], kind="stable")]
     nblk = (lv[:, 1] + qpb - 1) // qpb
     leaf = np.repeat(np.arange(lv.shape[0]), nblk)
     first = np.cumsum(nblk) - nblk
-    q0 = (np.arange(leaf.size) - first[leaf]) * qpb
-    items = np.stack([lv[leaf, 0], lv[leaf, 1], q0, np.zeros_like(q0)], axis=1)
-    items = items[np.argsort(-items[:, 1], kind="stable")]
-    return items.astype(np.int32).reshape(-1, 4)
+    items = np.zeros((leaf.size, 4), dtype=np.int32)
+    items[:, 0] = lv[leaf, 0]
+    items[:, 1] = lv[leaf, 1]
+    items[:, 2] = (np.arange(leaf.size) - first[leaf]) * qpb
+    return items
 
 
 def _cache_path():
@@ -174,8 +226,17 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "gsknn_tune.json"))
 
 
-def _launch(dtype, d, k, X, X2, perm, items, nitems, D, I, qpb, ct, grid, stream):
+def _launch(dtype, d, k, X, X2, perm, items, nitems, D, I, qpb, ct, grid, stream, mfma=None):
     L = gsknn.lib()
+    if mfma is not None:
+        xn, mx = mfma
+        rc = L.gsknn_sweep_leaves_mfma_sdev(d, k, gsknn._dp(X), gsknn._dp(X2), gsknn._dp(xn),
+                                            gsknn._dp(mx), gsknn._dp(perm), gsknn._dp(items),
+                                            gsknn._dp(nitems), gsknn._dp(D), gsknn._dp(I),
+                                            D.stride(0), qpb, grid, gsknn._stream_ptr(stream))
+        if rc:
+            raise gsknn.GsknnError(gsknn.last_error())
+        return
     fn = L.gsknn_sweep_leaves_ddev if dtype == "f64" else L.gsknn_sweep_leaves_sdev
     rc = fn(d, k, gsknn._dp(X), gsknn._dp(X2), gsknn._dp(perm), gsknn._dp(items),
             gsknn._dp(nitems), gsknn._dp(D), gsknn._dp(I), D.stride(0), qpb, ct, grid,
@@ -184,14 +245,15 @@ def _launch(dtype, d, k, X, X2, perm, items, nitems, D, I, qpb, ct, grid, stream
         raise gsknn.GsknnError(gsknn.last_error())
 
 
-def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None):
+def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None, mfma=None):
     """Pick (qpb, ct, grid) for the leaf sweep by timing each candidate on one
-    iteration's work list against scratch heaps.  Cached per shape."""
+    iteration's work list against scratch heaps.  Cached per shape.  With
+    mfma = (xn, mx) the FP32 MFMA form is tuned (qpb 32, ct unused: 0)."""
     import torch
     dev = X.device
     N, d = X.shape
     sizes = np.array([n for _, n in leaves])
-    key = f"{dtype}-d{d}-k{k}-N{N}-leaves{int(sizes.min())}-{int(sizes.max())}"
+    key = f"{dtype}{'-mfma' if mfma is not None else ''}-d{d}-k{k}-N{N}-leaves{int(sizes.min())}-{int(sizes.max())}"
     path = _cache_path()
     table = {}
     if cache and os.path.exists(path):
@@ -207,18 +269,18 @@ def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None):
     stream = torch.cuda.current_stream()
     elem = 8 if dtype == "f64" else 4
     results = []
-    for qpb in QPB_CHOICES:
+    for qpb in ((32,) if mfma is not None else QPB_CHOICES):
         items = torch.from_numpy(make_items(leaves, qpb)).to(dev)
         nitems = torch.tensor([items.shape[0]], dtype=torch.int32, device=dev)
-        for ct in (CT_CHOICES[:1] if d in SCALAR_DIMS else CT_CHOICES):
-            if int(gsknn.lib().gsknn_sweep_lds_bytes(elem, d, k, qpb, ct)) > 160 * 1024:
+        for ct in ((0,) if mfma is not None else CT_CHOICES[:1] if d in SCALAR_DIMS else CT_CHOICES):
+            if mfma is None and int(gsknn.lib().gsknn_sweep_lds_bytes(elem, d, k, qpb, ct)) > 160 * 1024:
                 continue
             for grid in GRID_CHOICES:
                 D.fill_(float("inf"))
                 I.fill_(-1)
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
-                _launch(dtype, d, k, X, X2, perm, items, nitems, D, I, qpb, ct, grid, stream)
+                _launch(dtype, d, k, X, X2, perm, items, nitems, D, I, qpb, ct, grid, stream, mfma)
                 e1.record(stream)
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1)
@@ -242,15 +304,23 @@ def autotune(X, X2, k, perm, leaves, dtype="f32", cache=True, log=None):
 class RPTreeSweep:
     """All-NN sweep state: points, global heap rows, the captured launch.
 
-    step() runs one iteration.  With prefetch (the default on a GPU) the
-    next iteration's tree is planned on the host and built on a second
-    stream while the current sweep runs, into one of two spare permutation
-    buffers, so an iteration costs about the larger of the two instead of
-    their sum.
+    step() runs one iteration.  Trees are built `batch` iterations at a time
+    (build_trees: one sort per level for all of them).  With prefetch (the
+    default on a GPU) a worker thread plans the next batch on the host (node
+    structure and work lists) and the batch is built on a second stream while
+    the current batch's sweeps run, into the other half of a double set of
+    permutation buffers; `iterations`, when given, stops building past that
+    many.  The first iteration, on empty heap rows, runs the exact VALU form
+    of the sweep (nearly every pair of a cold row needs its exact distance);
+    later iterations run the FP32 MFMA form where it applies (d in {16, 32},
+    k <= 64; exact_first=False runs it from iteration 0), launched once
+    directly and then replayed from a captured hipGraph.  Both forms give the
+    oracle's rows bit for bit.
     """
 
     def __init__(self, X, k, seed=3, leaf_lo=256, leaf_hi=4096, dtype="f32", config=None,
-                 use_graph=True, log=None, prefetch=True):
+                 use_graph=True, log=None, prefetch=True, batch=4, exact_first=True,
+                 iterations=None):
         import torch
         self.X = X
         self.k = k
@@ -263,84 +333,137 @@ class RPTreeSweep:
         tdt = torch.float64 if dtype == "f64" else torch.float32
         self.X2 = torch.empty(N, dtype=tdt, device=dev)
         gsknn.sqnorm_dev(X, self.X2)
+        # FP32 MFMA form where it applies (GSKNN_SWEEP_MFMA=0 keeps the exact
+        # VALU form, for comparison): both give the oracle's rows bit for bit.
+        self.mfma = None
+        if (dtype == "f32" and X.is_cuda and os.environ.get("GSKNN_SWEEP_MFMA", "1") != "0"
+                and gsknn.sweep_mfma_supported(d, k, 32)):
+            self.mfma = gsknn.sweep_norms_dev(X, self.X2)
         self.D = torch.full((N, k), float("inf"), dtype=tdt, device=dev)
         self.I = torch.full((N, k), -1, dtype=torch.int32, device=dev)
         self.perm = torch.empty(N, dtype=torch.int32, device=dev)
-        self.config = config
+        self.config = config          # exact VALU form: (qpb, ct, grid)
+        self.config_mfma = None       # MFMA form: (32, 0, grid)
         self.use_graph = use_graph
         self.graph = None
+        self._graph_mfma = None
         self.tune_info = None
+        self.tune_info_mfma = None
         self.iteration = 0
+        self.exact_first = exact_first
+        self.iterations = iterations
         self.timings = []
         self.prefetch = prefetch and X.is_cuda
-        self._bufs = [torch.empty(N, dtype=torch.int32, device=dev) for _ in range(2)]
+        self.batch = max(1, int(batch))
+        self._bufs = [[torch.empty(N, dtype=torch.int32, device=dev) for _ in range(self.batch)]
+                      for _ in range(2)]
         self._side = torch.cuda.Stream(device=dev) if self.prefetch else None
-        self._pending = None
-        self._last_end = None
+        self._pool = None
+        if self.prefetch:
+            from concurrent.futures import ThreadPoolExecutor
+            self._pool = ThreadPoolExecutor(max_workers=1)
+        self._plans = {}       # first iteration of a batch -> future of its plan
+        self._trees = {}       # iteration -> (buffer, leaves, batch record, items, qpb)
+        self._set_free = [None, None]  # event: the buffer set's last reader is done
 
-    def _build(self, it, after=None):
-        """Build tree `it` into spare buffer it % 2; on the side stream with
-        prefetch (after event `after`: the buffer's last reader is done).
-        Once the launch shape is known, the tree's work list is also made
-        here (on the host, while the current sweep runs) in pinned memory.
-        Returns (perm buffer, leaves, start event, done event, work list)."""
+    def _uses_mfma(self, it):
+        return self.mfma is not None and (it > 0 or not self.exact_first)
+
+    def _batch_its(self, it0):
+        end = it0 + self.batch
+        if self.iterations is not None:
+            end = min(end, self.iterations)
+        return list(range(it0, end))
+
+    def _plan_job(self, it0):
+        """Host planning of a batch: node structure, and the work list of
+        each tree whose launch shape is known already."""
+        its = self._batch_its(it0)
+        qpbs = [32 if self._uses_mfma(it) else (self.config[0] if self.config else None)
+                for it in its]
+        N, d = self.X.shape
+
+        def work():
+            plans = [plan_tree(N, d, it, self.seed, self.leaf_lo, self.leaf_hi) for it in its]
+            items = [make_items(pl[1], q) if q else None for pl, q in zip(plans, qpbs)]
+            return plans, items, qpbs
+        return self._pool.submit(work) if self._pool is not None else work
+
+    def _build_batch(self, it0):
+        """Build the trees of a batch into buffer set (it0 // batch) % 2, on
+        the side stream with prefetch (after that set's last copy); with
+        prefetch the next batch's planning starts on the worker thread."""
         import torch
-        buf = self._bufs[it % 2]
+        its = self._batch_its(it0)
+        if not its:
+            return
+        bset = (it0 // self.batch) % 2
         stream = self._side if self.prefetch else torch.cuda.current_stream()
         t0 = time.perf_counter()
-        plan = plan_tree(self.X.shape[0], self.X.shape[1], it, self.seed, self.leaf_lo,
-                         self.leaf_hi)
+        job = self._plans.pop(it0, None) or self._plan_job(it0)
+        plans, items, qpbs = job.result() if hasattr(job, "result") else job()
+        wait_ms = (time.perf_counter() - t0) * 1e3
         with torch.cuda.stream(stream):
-            if after is not None:
-                stream.wait_event(after)
+            if self._set_free[bset] is not None:
+                stream.wait_event(self._set_free[bset])
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            _, leaves = build_tree(self.X, it, self.seed, self.leaf_lo, self.leaf_hi, perm_out=buf,
-                                   plan=plan)
+            _, leaves = build_trees(self.X, its, self.seed, self.leaf_lo, self.leaf_hi,
+                                    perm_outs=self._bufs[bset][:len(its)], plans=plans)
             e1.record(stream)
-        items = None
-        if self.config is not None:
-            items = torch.from_numpy(make_items(leaves, self.config[0])).pin_memory()
-        self._plan_ms = (time.perf_counter() - t0) * 1e3
-        return buf, leaves, e0, e1, items
+        rec = {"e0": e0, "e1": e1, "n": len(its), "plan_wait_ms": wait_ms}
+        for t, it in enumerate(its):
+            self._trees[it] = (self._bufs[bset][t], leaves[t], rec, items[t], qpbs[t])
+        if self.prefetch:
+            nxt = it0 + self.batch
+            if self._batch_its(nxt):
+                self._plans[nxt] = self._plan_job(nxt)
 
-    def _ensure_config(self, perm, leaves):
-        if self.config is None:
-            self.config, self.tune_info = autotune(self.X, self.X2, self.k, perm, leaves,
-                                                   self.dtype, log=self.log)
-        qpb = self.config[0]
+    def _ensure_items(self):
         import torch
         if not hasattr(self, "items"):
             N = self.X.shape[0]
-            max_items = N // qpb + N // max(1, min(self.leaf_lo, qpb)) + 16
+            qmin = min(q for q in QPB_CHOICES + (32,))
+            max_items = N // qmin + N // max(1, min(self.leaf_lo, qmin)) + 16
             self.items = torch.zeros((max_items, 4), dtype=torch.int32, device=self.X.device)
             self.nitems = torch.zeros(1, dtype=torch.int32, device=self.X.device)
 
-    def _launch_captured(self):
+    def _launch(self, use_mfma):
         import torch
-        qpb, ct, grid = self.config
-        stream = torch.cuda.current_stream()
+        qpb, ct, grid = self.config_mfma if use_mfma else self.config
         _launch(self.dtype, self.X.shape[1], self.k, self.X, self.X2, self.perm, self.items,
-                self.nitems, self.D, self.I, qpb, ct, grid, stream)
+                self.nitems, self.D, self.I, qpb, ct, grid, torch.cuda.current_stream(),
+                self.mfma if use_mfma else None)
 
     def step(self):
         """One tree iteration: (wait for / build) its tree, rewrite the work
-        list, launch the sweep; with prefetch, start the next tree."""
+        list, launch the sweep; with prefetch, start the next tree batch."""
         import torch
         main = torch.cuda.current_stream()
         t0 = time.perf_counter()
-        if self._pending is None:
-            self._pending = self._build(self.iteration)
-        buf, leaves, et0, et1, items = self._pending
-        self._pending = None
-        if self.config is None:
+        it = self.iteration
+        if self.iterations is not None and it >= self.iterations:
+            raise ValueError(f"RPTreeSweep: {self.iterations} iterations were announced")
+        if it not in self._trees:
+            self._build_batch(it - it % self.batch)
+        buf, leaves, rec, items_np, items_qpb = self._trees.pop(it)
+        use_mfma = self._uses_mfma(it)
+        if (use_mfma and self.config_mfma is None) or (not use_mfma and self.config is None):
             torch.cuda.synchronize()  # the autotuner times launches on this tree
-        self._ensure_config(buf, leaves)
-        qpb = self.config[0]
-        if items is None:
-            items = torch.from_numpy(make_items(leaves, qpb)).pin_memory()
-        main.wait_event(et1)
+            if use_mfma:
+                self.config_mfma, self.tune_info_mfma = autotune(
+                    self.X, self.X2, self.k, buf, leaves, self.dtype, log=self.log,
+                    mfma=self.mfma)
+            else:
+                self.config, self.tune_info = autotune(self.X, self.X2, self.k, buf, leaves,
+                                                       self.dtype, log=self.log)
+        self._ensure_items()
+        qpb = (self.config_mfma if use_mfma else self.config)[0]
+        if items_np is None or items_qpb != qpb:
+            items_np = make_items(leaves, qpb)
+        items = torch.from_numpy(items_np).pin_memory()
+        main.wait_event(rec["e1"])
         self.perm.copy_(buf)
         copied = torch.cuda.Event()
         copied.record(main)
@@ -348,36 +471,48 @@ class RPTreeSweep:
         self.nitems.fill_(items.shape[0])
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(main)
-        if self.use_graph and self.graph is not None:
+        replayed = False
+        if self.use_graph and self.graph is not None and use_mfma == self._graph_mfma:
             self.graph.replay()
+            replayed = True
         else:
-            # First iteration (or no graph): launch directly; the direct
-            # launch also sets the kernel's attributes before any capture.
-            self._launch_captured()
+            # A form's first launch goes directly (it also sets the kernel's
+            # attributes before any capture).
+            self._launch(use_mfma)
         e1.record(main)
-        if self.prefetch:
-            # The next tree goes to the other buffer (read by the previous
-            # iteration's copy, long finished); it waits for this copy only
-            # so that the two streams never touch one buffer at once.  Its
-            # build may still run when this step returns: the next step's
-            # sweep waits for it on the device.
-            self._pending = self._build(self.iteration + 1, after=copied)
+        if it % self.batch == self.batch - 1 or it + 1 == self.iterations:
+            self._set_free[(it // self.batch) % 2] = copied
+        if self.prefetch and it % self.batch == 0:
+            # The next batch goes to the other buffer set, whose last copy
+            # (the previous batch's last step) is already enqueued; its build
+            # runs beside this batch's sweeps.
+            self._build_batch(it + self.batch)
         e1.synchronize()
         t1 = time.perf_counter()
         sweep_ms = e0.elapsed_time(e1)
-        tree_ms = et0.elapsed_time(et1)
-        if self.use_graph and self.graph is None:
-            # Capture the launch once; later iterations rewrite perm/items/
-            # nitems in place and replay it.  Capturing does not execute.
+        rec["e1"].synchronize()
+        tree_ms = rec["e0"].elapsed_time(rec["e1"]) / rec["n"]
+        if self.use_graph and not replayed and (self.graph is None or use_mfma != self._graph_mfma) \
+                and (use_mfma or self.mfma is None):
+            # Capture the steady form's launch once; later iterations rewrite
+            # perm/items/nitems in place and replay it.  Capturing does not
+            # execute.
             self.graph = torch.cuda.CUDAGraph()
+            self._graph_mfma = use_mfma
             with torch.cuda.graph(self.graph):
-                self._launch_captured()
+                self._launch(use_mfma)
             torch.cuda.synchronize()
         pairs = float(sum(n * n for _, n in leaves))
-        info = {"iteration": self.iteration, "tree_ms": tree_ms, "sweep_ms": sweep_ms,
-                "tree_plan_ms": self._plan_ms,
-                "iteration_ms": (t1 - t0) * 1e3, "leaves": len(leaves), "pairs": pairs,
-                "leaf_sizes": sorted({n for _, n in leaves})}
+        info = {"iteration": it, "tree_ms": tree_ms, "sweep_ms": sweep_ms,
+                "plan_wait_ms": rec["plan_wait_ms"], "form": "mfma" if use_mfma else "exact",
+                "graph": replayed, "iteration_ms": (t1 - t0) * 1e3, "leaves": len(leaves),
+                "pairs": pairs, "leaf_sizes": sorted({n for _, n in leaves})}
         self.timings.append(info)
         self.iteration += 1
         return buf, leaves, info
+
+    def close(self):
+        """Wait for the worker thread (planning ahead) to finish."""
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None

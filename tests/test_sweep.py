@@ -65,9 +65,11 @@ def test_items_cover_each_leaf_once():
 
 @pytest.mark.gpu
 def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
-    # Iteration 0 launches directly; iterations 1-3 replay the captured graph
-    # after eager tree builds.  The heap rows are checked after every
-    # iteration against the composition of per-leaf oracle calls.
+    # Iteration 0 runs the exact form directly on the empty rows; iteration 1
+    # the MFMA form directly (then captured), iterations 2-4 replay it; the
+    # fifth iteration's tree comes from the second batch of four, built on
+    # the side stream.  The heap rows are checked after every iteration
+    # against the composition of per-leaf oracle calls.
     import torch
     monkeypatch.setenv("GSKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
     dev = torch.device("cuda")
@@ -79,8 +81,10 @@ def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
     assert np.array_equal(sw.X2.cpu().numpy(), X2)
     Dr = np.full((N, k), np.inf, dtype=np.float32)
     Ir = np.full((N, k), -1, dtype=np.int32)
-    for _ in range(4):
+    forms = []
+    for _ in range(5):
         perm, leaves, info = sw.step()
+        forms.append((info["form"], info["graph"]))
         print(f"[gpu] sweep iteration {info['iteration']}: {info['sweep_ms']:.3f} ms", flush=True)
         perm = perm.cpu().numpy()
         for s, n in leaves:
@@ -93,6 +97,24 @@ def test_sweep_matches_per_leaf_oracle(tmp_path, monkeypatch):
         np.testing.assert_array_equal(a[1], b[1])
         assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
     assert sw.graph is not None
+    assert forms == [("exact", False), ("mfma", False), ("mfma", True), ("mfma", True),
+                     ("mfma", True)]
+
+
+@pytest.mark.gpu
+def test_batched_trees_equal_single_trees():
+    # Several trees sorted together per level (one key space numbering the
+    # nodes of all trees) give each tree's single build, permutation and
+    # leaves alike.
+    import torch
+    dev = torch.device("cuda")
+    X = torch.from_numpy(ds.config4_points(20000, 16, 32, seed=5)).to(dev)
+    print("[gpu] batched trees N=20000 d=16", flush=True)
+    perms, leaves = sweep.build_trees(X, [3, 4, 5, 6, 7], seed=11, leaf_lo=100, leaf_hi=900)
+    for t, it in enumerate([3, 4, 5, 6, 7]):
+        p1, l1 = sweep.build_tree(X, it, seed=11, leaf_lo=100, leaf_hi=900)
+        assert l1 == leaves[t]
+        assert torch.equal(p1, perms[t])
 
 
 @pytest.mark.gpu
@@ -164,3 +186,40 @@ def test_device_tree_partition_and_median():
         proj = (proj + Y[:, q] * direc[q]).astype(np.float32)
     order = sorted(range(200), key=lambda i: (proj[i], i))
     assert list(perm.cpu().numpy()[:100]) == sorted(order[:100], key=lambda i: (proj[i], i))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,d,k,lo,hi,mfma,exact_first",
+                         [(10000, 16, 20, 40, 700, "1", True), (9000, 32, 7, 100, 900, "1", False),
+                          (7000, 32, 64, 80, 600, "1", False), (6000, 32, 32, 64, 512, "0", True)])
+def test_sweep_odd_leaves_both_forms(tmp_path, monkeypatch, N, d, k, lo, hi, mfma, exact_first):
+    """Leaf sizes that are not multiples of the tile or query block (N not a
+    power of two), odd k; the FP32 MFMA form (from the empty rows on, with
+    exact_first=False) and (GSKNN_SWEEP_MFMA=0) the exact VALU form, three
+    iterations each, bitwise against the composition of per-leaf oracle
+    calls."""
+    import torch
+    monkeypatch.setenv("GSKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setenv("GSKNN_SWEEP_MFMA", mfma)
+    dev = torch.device("cuda")
+    Xn = ds.config4_points(N, d, 40, seed=5)
+    X = torch.from_numpy(Xn).to(dev)
+    sw = sweep.RPTreeSweep(X, k, seed=11, leaf_lo=lo, leaf_hi=hi, exact_first=exact_first)
+    assert (sw.mfma is not None) == (mfma == "1")
+    X2 = oracle.sqnorm(Xn)
+    Dr = np.full((N, k), np.inf, dtype=np.float32)
+    Ir = np.full((N, k), -1, dtype=np.int32)
+    for _ in range(3):
+        perm, leaves, info = sw.step()
+        print(f"[gpu] sweep N={N} d={d} k={k} mfma={mfma} iteration {info['iteration']}", flush=True)
+        assert any(n % 32 for _, n in leaves)
+        perm = perm.cpu().numpy()
+        for s, n in leaves:
+            ids = np.ascontiguousarray(perm[s:s + n])
+            Ds, Is = np.ascontiguousarray(Dr[ids]), np.ascontiguousarray(Ir[ids])
+            oracle.gsknn(Xn, X2, ids, Xn, X2, ids, Ds, Is, canonical=True)
+            Dr[ids], Ir[ids] = Ds, Is
+        a = oracle.sort_rows(sw.D.cpu().numpy(), sw.I.cpu().numpy())
+        b = oracle.sort_rows(Dr, Ir)
+        np.testing.assert_array_equal(a[1], b[1])
+        assert np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8))
