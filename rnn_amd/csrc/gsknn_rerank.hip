@@ -166,6 +166,22 @@ __device__ __forceinline__ size_t list_base(const Bf16Args &a, int sp, int gg, i
 // is row kRowOf(l, v), column l%16.
 __device__ __forceinline__ int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
 
+// v[bb] for a per-lane bb in [0, N): a tree of selects on the bits of bb
+// (N - 1 selects).  The values must be scalars (FP32 results): a select
+// between two elements of one vector would become an element at a run-time
+// index, a chain of compares per level.
+template <int LO, int N, int NV>
+__device__ __forceinline__ float select_tree(const float (&v)[NV], int bb) {
+  if constexpr (N == 1) {
+    return v[LO];
+  } else {
+    constexpr int H = N / 2;
+    const float lo = select_tree<LO, H, NV>(v, bb);
+    const float hi = select_tree<LO + H, H, NV>(v, bb);
+    return (bb & H) ? hi : lo;
+  }
+}
+
 struct FilterLayout {
   size_t tA, nc, xn, en, id, heap, cnt, bytes;
   __host__ __device__ FilterLayout(int KS, int CB, int Q, int k) {
@@ -761,11 +777,13 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   // and +inf marks an empty place.
   const int kg = (k + 3 - grp) / 4;
   int hu[QB][RK];
-  auto insert = [&](int qb, float xf) __attribute__((always_inline)) {
-    const int x = __float_as_int(xf);
+  auto insert_bits = [&](int qb, int x) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = RK - 1; i >= 1; --i) hu[qb][i] = max(hu[qb][i - 1], min(hu[qb][i], x));
     hu[qb][0] = min(hu[qb][0], x);
+  };
+  auto insert = [&](int qb, float xf) __attribute__((always_inline)) {
+    insert_bits(qb, __float_as_int(xf));
   };
   constexpr int kNegInf = (int)0xFF800000u, kPosInf = 0x7F800000;
 #pragma unroll
@@ -875,7 +893,9 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
           acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bq[qb][s], acc[qb][cb], 0, 0, 0);
       }
     }
-    // Hot path: one FMA and one min per pair.
+    // Hot path: one FMA and one min per pair (the values as scalars, slot
+    // cb 4 + v).
+    float tq[QB][CB * 4];
     float tmin[QB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
@@ -886,7 +906,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
         for (int v = 0; v < 4; ++v) {
           const int c = cb * 16 + out_row(lane, v);
           const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
-          acc[qb][cb][v] = tt;
+          tq[qb][cb * 4 + v] = tt;
           tmin[qb] = fminf(tmin[qb], tt);
         }
     }
@@ -899,40 +919,41 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
         unsigned bits = 0u;
         if (tmin[qb] <= thr[qb]) {
 #pragma unroll
-          for (int bb = 0; bb < CB * 4; ++bb)
-            bits |= (acc[qb][bb >> 2][bb & 3] <= thr[qb] ? 1u : 0u) << bb;
+          for (int bb = 0; bb < CB * 4; ++bb) bits |= (tq[qb][bb] <= thr[qb] ? 1u : 0u) << bb;
         }
         const int j = qbase + ql[qb];
         const size_t lb = ((size_t)split * a.n + j) * a.cap + (size_t)grp * capg;
         // Wave-uniform loop: the record shuffles read other lanes, which
-        // must be active (a shuffle from an inactive lane returns 0).
-        while (__any(bits != 0u)) {
+        // must be active (a shuffle from an inactive lane returns 0).  The
+        // body is straight-line (a lane without a survivor inserts +inf, which
+        // leaves its set unchanged), so the sets update in place.
+        if (__any(bits != 0u)) do {
           const bool act = bits != 0u;
           const int bb = act ? __builtin_ctz(bits) : 0;
           bits &= bits - 1;
-          float tt = acc[qb][0][0];
-#pragma unroll
-          for (int b2 = 1; b2 < CB * 4; ++b2)
-            if (bb == b2) tt = acc[qb][b2 >> 2][b2 & 3];
+          const float tt = select_tree<0, CB * 4, CB * 4>(tq[qb], bb);
           const int c = (bb >> 2) * 16 + out_row(lane, bb & 3);
           // This tile's bound terms (finish_tile of the next tile, after
           // the survivors, replaces them).
           const float rx = __shfl(rc_x, c), re = __shfl(rc_e, c), rn = __shfl(rc_n, c);
           const int rid = __shfl(rc_i, c);
-          if (!act || cbeg + t * C + c >= cend) continue;
+          const bool valid = act && cbeg + t * C + c < cend;
           const float dist = tt + nq[qb];
           const float delta = (rx * al[qb] + re * be[qb] + 4.0f * u * rn + cq[qb]) *
                                   1.0009765625f + 1e-30f;
           const float lower = dist - delta, upper = dist + delta;
-          if (lower <= uq[qb]) {
-            const int at = lc[qb]++;
-            if (at < capg) {
-              a.lst[lb + at] = make_int2(rid, __float_as_int(lower));
-              a.lstu[lb + at] = upper;
-            }
+          const bool app = valid && lower <= uq[qb];
+          const int at = lc[qb];
+          lc[qb] = at + (app ? 1 : 0);
+          if (app && at < capg) {
+            a.lst[lb + at] = make_int2(rid, __float_as_int(lower));
+            a.lstu[lb + at] = upper;
           }
-          if (__float_as_int(upper) < hu[qb][RK - 1]) insert(qb, upper);
-        }
+          // max with +inf's pattern for a lane without a valid survivor (an
+          // integer max of constants, which the compiler keeps branch-free).
+          const int ux = max(__float_as_int(upper), valid ? (int)0x80000000 : kPosInf);
+          insert_bits(qb, ux);
+        } while (__any(bits != 0u));
       }
       refresh();
     }
