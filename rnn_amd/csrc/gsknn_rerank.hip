@@ -952,7 +952,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
           // max with +inf's pattern for a lane without a valid survivor (an
           // integer max of constants, which the compiler keeps branch-free).
           const int ux = max(__float_as_int(upper), valid ? (int)0x80000000 : kPosInf);
-          insert_bits(qb, ux);
+          if (__any(ux < hu[qb][RK - 1])) insert_bits(qb, ux);
         } while (__any(bits != 0u));
       }
       refresh();
