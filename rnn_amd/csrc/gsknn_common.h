@@ -12,6 +12,16 @@
 
 namespace gsknn {
 
+// Pass threshold of the FP32 and BF16 candidate filters.  A pair whose MFMA
+// value t = fl(-2 dot~ + |c|^2) satisfies |(t + nq) - dist| <= delta <= dmx
+// for the distance dist it brackets can reach the bound U only if
+// t <= (U - nq) + dmx in exact arithmetic; the FP32 evaluation is raised by
+// 2^-18 (|U| + |nq| + dmx), which covers its own three roundings and those
+// of the pair's bounds (2^-20 of |U| + 3 dmx), so every such pair passes.
+__device__ __forceinline__ float f32_pass_thr(float U, float nq, float dmx) {
+  return __builtin_fmaf(0x1p-18f, fabsf(U) + fabsf(nq) + dmx, (U - nq) + dmx);
+}
+
 // Kernel arguments of one kNN launch.  All pointers are device pointers.
 template <typename T>
 struct KnnArgs {

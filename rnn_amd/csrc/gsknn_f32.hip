@@ -36,6 +36,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr float kU = 5.9604644775390625e-08f;  // 2^-24
 
+// x moved outward by 2^-22 |x|, so that a bound computed with round-to-nearest
+// additions (each off by at most 2^-24 of its result) still holds.
+__device__ __forceinline__ float widen_up(float x) { return __builtin_fmaf(0x1p-22f, fabsf(x), x); }
+__device__ __forceinline__ float widen_dn(float x) { return __builtin_fmaf(-0x1p-22f, fabsf(x), x); }
+
 __device__ __forceinline__ bool cl_f(float da, int ia, float db, int ib) {
   return da < db || (da == db && ia < ib);
 }
@@ -313,7 +318,7 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
   float thr[QB];
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb)
-    thr[qb] = qv[qb] ? hv[ql[qb]] - nq[qb] + dmx[qb] : -__builtin_inff();
+    thr[qb] = qv[qb] ? f32_pass_thr(hv[ql[qb]], nq[qb], dmx[qb]) : -__builtin_inff();
 
   float pq_lo[QR], pq_up[QR];
   int pq_ps[QR], pq_q[QR];
@@ -334,7 +339,7 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
   auto refresh = [&]() {
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb)
-      if (qv[qb]) thr[qb] = hv[ql[qb]] - nq[qb] + dmx[qb];
+      if (qv[qb]) thr[qb] = f32_pass_thr(hv[ql[qb]], nq[qb], dmx[qb]);
   };
   auto flush = [&]() {
 #pragma unroll 1
@@ -423,13 +428,13 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
               const float dist = tt + nq[qb];
               const float delta = txn[c] * al[qb] + 4.0f * u * fabsf(tnc[c]) + cq[qb] + 1e-30f;
               if (direct) {
-                apply(ql[qb], dist - delta, dist + delta, cbeg + t * C + c);
+                apply(ql[qb], widen_dn(dist - delta), widen_up(dist + delta), cbeg + t * C + c);
               } else {
 #pragma unroll
                 for (int r = 0; r < QR; ++r)
                   if (pq_n == r) {
-                    pq_lo[r] = dist - delta;
-                    pq_up[r] = dist + delta;
+                    pq_lo[r] = widen_dn(dist - delta);
+                    pq_up[r] = widen_up(dist + delta);
                     pq_ps[r] = cbeg + t * C + c;
                     pq_q[r] = qb;
                   }

@@ -57,6 +57,11 @@ __device__ __forceinline__ bool cless_d(double da, int ia, double db, int ib) {
   return da < db || (da == db && ia < ib);
 }
 
+// x moved outward by 2^-22 |x|, so that a bound computed with round-to-nearest
+// additions (each off by at most 2^-24 of its result) still holds.
+__device__ __forceinline__ float widen_up(float x) { return __builtin_fmaf(0x1p-22f, fabsf(x), x); }
+__device__ __forceinline__ float widen_down(float x) { return __builtin_fmaf(-0x1p-22f, fabsf(x), x); }
+
 }  // namespace
 
 // BF16 row length: d rounded up to 32 KS dims, KS = 1, 2, 4 or 8 (the
@@ -151,6 +156,7 @@ struct Bf16Args {
   int *cnt;     // [nsplit][G][n] entries appended to each list
   float *U;     // [nsplit][n]
   int *nover;   // lists that overflowed
+  int warm;     // register form: tiles of each split in the warm-up pass
 };
 
 // List gg of split sp of query j: its count and its first entry.
@@ -327,7 +333,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
     al[qb] = 2.0f * eq + cxx * xq;
     be[qb] = 2.0f * xq + 2.0f * eq;
     cq[qb] = 4.0f * u * nq[qb];
-    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f;
+    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f + 1e-30f;
   }
 
   const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
@@ -437,13 +443,13 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     uq[qb] = group_u(qb);
-    thr[qb] = qv[qb] ? uq[qb] - nq[qb] + dmx[qb] : -__builtin_inff();
+    thr[qb] = qv[qb] ? f32_pass_thr(uq[qb], nq[qb], dmx[qb]) : -__builtin_inff();
   }
   auto refresh = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
       uq[qb] = group_u(qb);
-      if (qv[qb]) thr[qb] = uq[qb] - nq[qb] + dmx[qb];
+      if (qv[qb]) thr[qb] = f32_pass_thr(uq[qb], nq[qb], dmx[qb]);
     }
   };
 
@@ -578,7 +584,7 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
               const float dist = tt + nq[qb];
               const float delta = (txn[c] * al[qb] + ten[c] * be[qb] + 4.0f * u * tnc[c] + cq[qb]) *
                                       1.0009765625f + 1e-30f;
-              apply(qb, dist - delta, dist + delta, tcid[c]);
+              apply(qb, widen_down(dist - delta), widen_up(dist + delta), tcid[c]);
             }
           }
         });
@@ -602,8 +608,8 @@ __attribute__((amdgpu_waves_per_eu(bf16_waves_per_eu(KS, QB)))) void knn_bf16_fi
 #pragma unroll
             for (int r = 0; r < QR; ++r)
               if (pq_n == r) {
-                pq_lo[r] = dist - delta;
-                pq_up[r] = dist + delta;
+                pq_lo[r] = widen_down(dist - delta);
+                pq_up[r] = widen_up(dist + delta);
                 pq_ps[r] = tcid[c];
                 pq_q[r] = qb;
               }
@@ -736,7 +742,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   const __bf16 *xg = (const __bf16 *)(a.panel + PL.rows);
   const float4 *grec = (const float4 *)(a.panel + PL.rec);
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, grp = lane >> 4, si = lane & 15;
+  const int tid = threadIdx.x, lane = tid & 63, grp = lane >> 4, si = lane & 15;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int qbase = blockIdx.x * Q;
   const int split = blockIdx.y;
   const int cbeg = split * a.cps;
@@ -767,7 +774,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
     al[qb] = 2.0f * eq + cxx * xq;
     be[qb] = 2.0f * xq + 2.0f * eq;
     cq[qb] = 4.0f * u * nq[qb];
-    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f;
+    // At least every candidate's delta.
+    dmx[qb] = (mxA[0] * al[qb] + mxA[1] * be[qb] + 4.0f * u * mxA[2] + cq[qb]) * 1.002f + 1e-30f;
   }
 
   // Upper-bound sets, seeded with the resident row rounded up to FP32.  The
@@ -807,7 +815,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
       uq[qb] = group_u(qb);
-      thr[qb] = qv[qb] ? uq[qb] - nq[qb] + dmx[qb] : -__builtin_inff();
+      thr[qb] = qv[qb] ? f32_pass_thr(uq[qb], nq[qb], dmx[qb]) : -__builtin_inff();
     }
   };
   refresh();
@@ -831,14 +839,22 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
     const int bidx = w * BPW + e;
     doff[e] = (unsigned)(((bidx % CB) * 16 + si) * DP * 2 + ((bidx / CB) * 32 + 8 * grp) * 2);
   }
-  auto bufp = [&](int t) -> unsigned char * { return smem + (size_t)(t & 1) * L.buf; };
+  // The launch walks a step sequence: steps 0 .. T0 - 1 are the warm-up pass
+  // over tiles 0 .. T0 - 1, steps T0 .. T0 + ntiles - 1 the main pass over
+  // every tile.  Step v uses LDS buffer v & 1.
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
+  const int T0 = min(a.warm, ntiles);
+  const int nsteps = ntiles > 0 ? T0 + ntiles : 0;
+  auto tile_of = [&](int v) { return v < T0 ? v : v - T0; };
+  auto bufp = [&](int v) -> unsigned char * { return smem + (size_t)(v & 1) * L.buf; };
   // Record of candidate `lane` of the next tile, held as loaded until the
   // tile's loads have landed, and the bound terms of the current tile.
   float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
   float rc_x = 0.f, rc_e = 0.f, rc_n = 0.f;
   int rc_i = 0;
-  auto issue_tile = [&](int t) __attribute__((always_inline)) {
-    unsigned char *b = bufp(t);
+  auto issue_tile = [&](int v) __attribute__((always_inline)) {
+    unsigned char *b = bufp(v);
+    const int t = tile_of(v);
     const char *rows = tile_rows(t);
 #pragma unroll
     for (int e = 0; e < BPW; ++e) {
@@ -851,9 +867,9 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   };
   // After the wave's loads of tile t have landed: the bound terms become
   // current, and wave w writes the norms of block w.
-  auto finish_tile = [&](int t) __attribute__((always_inline)) {
-    unsigned char *b = bufp(t);
-    const bool ok = cbeg + t * C + lane < cend;
+  auto finish_tile = [&](int v) __attribute__((always_inline)) {
+    unsigned char *b = bufp(v);
+    const bool ok = cbeg + tile_of(v) * C + lane < cend;
     rc_n = ok ? st.x : __builtin_inff();
     rc_x = ok ? st.y : 0.0f;
     rc_e = ok ? st.z : 0.0f;
@@ -864,18 +880,18 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
   // LDS loads included) has completed.
   auto wait_loads = []() __attribute__((always_inline)) { __builtin_amdgcn_s_waitcnt(0x0F70); };
 
-  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
-  if (ntiles > 0) {
+  if (nsteps > 0) {
     issue_tile(0);
     wait_loads();
     finish_tile(0);
   }
 
-  for (int t = 0; t < ntiles; ++t) {
-    __syncthreads();  // tile t in LDS; every wave done with tile t - 1
-    const bool more = t + 1 < ntiles;
-    if (more) issue_tile(t + 1);
-    const unsigned char *b = bufp(t);
+  for (int step = 0; step < nsteps; ++step) {
+    __syncthreads();  // this step's tile in LDS; every wave done with the last step
+    const int t = tile_of(step);
+    const bool more = step + 1 < nsteps;
+    if (more) issue_tile(step + 1);
+    const unsigned char *b = bufp(step);
     const __bf16 *tA = (const __bf16 *)(b + L.tile);
     const float *tnc = (const float *)(b + L.nc);
     f32x4 acc[QB][CB];
@@ -910,9 +926,33 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
           tmin[qb] = fminf(tmin[qb], tt);
         }
     }
+    if (step < T0) {
+      // Warm-up step: each lane's smallest tt of the tile (its 16 slots, a
+      // chunk of candidates no other lane or step sees) gives an upper bound
+      // of that one candidate's distance, (tmin + nq) + delta_max rounded
+      // up; the sets take these bounds.  Every valid lane inserts at once:
+      // no survivor loop, no list entries.
+      int xb[QB];
+      bool need = false;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        const float s = (tmin[qb] + nq[qb]) + dmx[qb];
+        xb[qb] = qv[qb] ? __float_as_int(widen_up(s)) : kPosInf;
+        need |= xb[qb] < hu[qb][RK - 1];
+      }
+      if (__any(need)) {
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) insert_bits(qb, xb[qb]);
+        refresh();
+      }
+    }
+    // Main pass.  Over tiles the warm-up pass covered, survivors are listed
+    // but their upper bounds stay out of the sets (each set entry stands for
+    // a distinct candidate).
+    const bool ins = step >= 2 * T0;
     bool hit = false;
 #pragma unroll
-    for (int qb = 0; qb < QB; ++qb) hit |= tmin[qb] <= thr[qb];
+    for (int qb = 0; qb < QB; ++qb) hit |= step >= T0 && tmin[qb] <= thr[qb];
     if (__any(hit)) {
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
@@ -941,7 +981,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
           const float dist = tt + nq[qb];
           const float delta = (rx * al[qb] + re * be[qb] + 4.0f * u * rn + cq[qb]) *
                                   1.0009765625f + 1e-30f;
-          const float lower = dist - delta, upper = dist + delta;
+          const float lower = widen_down(dist - delta), upper = widen_up(dist + delta);
           const bool app = valid && lower <= uq[qb];
           const int at = lc[qb];
           lc[qb] = at + (app ? 1 : 0);
@@ -951,7 +991,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
           }
           // max with +inf's pattern for a lane without a valid survivor (an
           // integer max of constants, which the compiler keeps branch-free).
-          const int ux = max(__float_as_int(upper), valid ? (int)0x80000000 : kPosInf);
+          const int ux = max(__float_as_int(upper), valid && ins ? (int)0x80000000 : kPosInf);
           if (__any(ux < hu[qb][RK - 1])) insert_bits(qb, ux);
         } while (__any(bits != 0u));
       }
@@ -959,7 +999,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_bf16_filter2_kernel(Bf16Args a)
     }
     if (more) {
       wait_loads();
-      finish_tile(t + 1);
+      finish_tile(step + 1);
     }
   }
 
@@ -1718,6 +1758,11 @@ hipError_t launch_knn_bf16(int m, int n, int d, int k, int NA, int NB, const flo
   a.cnt = (int *)((char *)work + o_cnt);
   a.U = (float *)((char *)work + o_u);
   a.nover = (int *)((char *)work + o_over);
+  // Warm-up pass of the register form: the first 1/16 of each split's tiles
+  // (GSKNN_BF16_WARM=<tiles>, read at each call, sets the count).
+  const char *warm_e = getenv("GSKNN_BF16_WARM");
+  const int warm_env = warm_e ? atoi(warm_e) : -1;
+  a.warm = p.reg ? (warm_env >= 0 ? warm_env : p.cps / (16 * kCBb) / 16) : 0;
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, a.nover);
   if (p.reg)
     hipLaunchKernelGGL(bf16_gather_kernel, dim3((m + kPanelPad + 3) / 4), dim3(256), 0, s, m, NA, d, amap,

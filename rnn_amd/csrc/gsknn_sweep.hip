@@ -303,7 +303,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void kn
     auto refresh = [&]() __attribute__((always_inline)) {
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb)
-        thr[qb] = qv[qb] ? hd[ql[qb]] - nq[qb] + dmx[qb] : -__builtin_inff();
+        thr[qb] = qv[qb] ? f32_pass_thr(hd[ql[qb]], nq[qb], dmx[qb]) : -__builtin_inff();
     };
     refresh();
 

@@ -165,3 +165,26 @@ def test_rows_padded_to_filter_k_steps(d, k):
     Dr, Ir, X2 = _oracle(X, amaps, bmap, k)
     Is = _sorted(Dr, Ir)[1]
     assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
+
+
+@pytest.mark.parametrize("warm", [0, 1, 5, 18, 40])
+def test_register_form_warm_up_pass(warm, monkeypatch):
+    # The register form's warm-up pass over the first tiles of each split
+    # (upper bounds from each lane's smallest value per tile), then the main
+    # pass over every tile; 0 = none, 18 = the whole split (7 splits of 18
+    # tiles here), 40 = more than the split holds.  Carried-over rows.
+    g = np.random.Generator(np.random.PCG64(3000 + warm))
+    N = 12000
+    X = ds.low_rank(N, 256, 10, seed=12)
+    perm = g.permutation(N).astype(np.int32)
+    amaps = [perm[:700].copy(), perm[700:8700].copy()]
+    bmap = perm[9000:].copy()
+    k = 64
+    monkeypatch.setenv("GSKNN_BF16_WARM", str(warm))
+    print(f"[gpu] bf16 register form warm-up {warm}", flush=True)
+    D, I = ds.empty_heap(bmap.size, k)
+    Dg, Ig, over = _run(X, amaps, bmap, D, I)
+    assert over == 0
+    Dr, Ir, X2 = _oracle(X, amaps, bmap, k)
+    Is = _sorted(Dr, Ir)[1]
+    assert_rows(Dg, Ig, Dr, Ir, X2q=X2[bmap], X2c_rows=X2[Is], tol=TOL64)
