@@ -851,9 +851,33 @@ __device__ __forceinline__ double dpp_move(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Candidates per lane per staged chunk: LDS holds the [d][64 CH] tile
-// (CH d <= 128: at most 64 KiB for FP64) plus the four query rows.
+// Candidates per lane per staged chunk: LDS holds the [d][64 CH + 4] tile
+// (CH d <= 128: at most 68 KiB for FP64), the chunk's norms and the four
+// query rows.
 constexpr int small_ch(int d) { return d <= 16 ? 8 : d <= 32 ? 4 : 2; }
+constexpr int small_stride(int ch) { return 64 * ch + 4; }
+
+// Bitonic sort of one (distance, index) entry per lane across the wave,
+// ascending in canonical order (lane r ends with the r-th smallest).
+template <typename T>
+__device__ __forceinline__ void wave_sort_asc(T &d, int &i, int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const T od = __shfl_xor(d, stride);
+      const int oi = __shfl_xor(i, stride);
+      // Ascending blocks (lane & size == 0) keep the smaller entry in the
+      // lower lane of each pair, descending blocks the larger.
+      const bool keep_min = ((lane & size) == 0) == ((lane & stride) == 0);
+      const bool take = keep_min ? cless(od, oi, d, i) : cless(d, i, od, oi);
+      if (take) {
+        d = od;
+        i = oi;
+      }
+    }
+  }
+}
 
 // Offer entry (x, xi) at slot xs to a lane's two smallest entries so far,
 // (bd, bi, bs) <= (sd, si, ss) in canonical order; a slot of -1 is empty.
@@ -874,9 +898,13 @@ template <typename T, int P, int CH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void knn_small_m_kernel(KnnArgs<T> a) {
   static_assert(P <= 32 && P % CH == 0, "candidate slots per lane");
   constexpr int C = 64 * CH;  // candidates per chunk
+  // Row stride of the staged tile: C + 4 puts the 16 dims a staging store
+  // writes (one row apart) in different LDS banks.
+  constexpr int S = small_stride(CH);
   extern __shared__ __align__(16) unsigned char smem[];
-  T *xs = (T *)smem;           // [d][C]
-  T *qs = xs + (size_t)C * a.d;  // [4][64]: the query rows
+  T *xs = (T *)smem;           // [d][S]
+  T *xn = xs + (size_t)S * a.d;  // [C]: the chunk's candidate norms XA2
+  T *qs = xn + C;                // [4][64]: the query rows
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int j = blockIdx.x * 4 + w;
   const bool qok = j < a.n;  // waves past n still help stage
@@ -893,41 +921,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   T v[P];
 #pragma unroll
   for (int i = 0; i < P; ++i) v[i] = T(0);
+  // Staging: 16 threads per row, thread t reading dims t % 16, t % 16 + 16,
+  // ... (NH column groups cover d <= 16 NH) of rows rr, rr + 16, ... (R
+  // rows), and the norms of its rows r with r % 16 == t % 16.  The ids of a
+  // chunk's rows are read while the previous chunk is staged and summed, so
+  // a chunk costs one round trip (rows past m read row 0, never used).
+  constexpr int R = C / 16, NH = 8 / CH;
+  static_assert(R * NH == 32, "32 staged values per thread and chunk");
+  const int p0 = tid & 15, rr = tid >> 4;
+  constexpr int NE = (R + 15) / 16;
+  int rid[R], nid[NE];  // the thread's rows; the rows whose norms it stages
+  auto load_ids = [&](int c0) __attribute__((always_inline)) {
 #pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int c = c0 + r * 16 + rr;
+      rid[r] = c < m ? a.amap[c] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int c = c0 + (p0 + 16 * e) * 16 + rr;
+      nid[e] = p0 + 16 * e < R && c < m ? a.amap[c] : 0;
+    }
+  };
+  load_ids(0);
+#pragma clang loop unroll(full)
   for (int ch = 0; ch < P / CH; ++ch) {
     const int c0 = ch * C;
     if (c0 < m) {  // uniform
-      // Stage rows c0 .. c0 + C - 1 (rows past m read row 0, never used):
-      // 16 threads per row, thread t reading dims t % 16, t % 16 + 16, ...,
-      // 16 rows per pass and eight passes in flight at a time.
-      const int p0 = tid & 15, rr = tid >> 4;
+      T x[R][NH];
+      T xq[NE];
 #pragma unroll
-      for (int g = 0; g < C / 128; ++g) {
-        int rid[8];
+      for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = c0 + (g * 8 + u) * 16 + rr;
-          rid[u] = c < m ? a.amap[c] : 0;
-        }
+        for (int h = 0; h < NH; ++h)
+          if (p0 + 16 * h < d) x[r][h] = a.XA[(size_t)rid[r] * d + p0 + 16 * h];
 #pragma unroll
-        for (int h = 0; h < 64; h += 16) {
-          if (p0 + h < d) {
-            T x[8];
+      for (int e = 0; e < NE; ++e) xq[e] = a.XA2[nid[e]];
+      if (c0 + C < m) load_ids(c0 + C);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = a.XA[(size_t)rid[u] * d + p0 + h];
+      for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) xs[(p0 + h) * C + (g * 8 + u) * 16 + rr] = x[u];
-          }
-        }
-      }
+        for (int h = 0; h < NH; ++h)
+          if (p0 + 16 * h < d) xs[(p0 + 16 * h) * S + r * 16 + rr] = x[r][h];
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+        if (p0 + 16 * e < R) xn[(p0 + 16 * e) * 16 + rr] = xq[e];
       __syncthreads();
       if (qok) {
+#pragma unroll 4
         for (int p = 0; p < d; ++p) {
           const T qp = qs[w * 64 + p];
 #pragma unroll
           for (int u = 0; u < CH; ++u)
-            v[ch * CH + u] = xadd(v[ch * CH + u], xmul(xs[p * C + lane + 64 * u], qp));
+            v[ch * CH + u] = xadd(v[ch * CH + u], xmul(xs[p * S + lane + 64 * u], qp));
         }
+        // The oracle's epilogue: x(-2), + XA2[amap[i]] (the query norm later).
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+          v[ch * CH + u] = xadd(xmul(v[ch * CH + u], T(-2)), xn[lane + 64 * u]);
       }
       __syncthreads();
     }
@@ -940,10 +991,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   bool rtk = false;
 #pragma unroll
   for (int i = 0; i < P; ++i) {
-    if (lane + 64 * i < m)
-      v[i] = xadd(xadd(xmul(v[i], T(-2)), a.XA2[id[i]]), qn);
-    else
+    if (lane + 64 * i < m) {
+      v[i] = xadd(v[i], qn);
+    } else {
       tk |= 1u << i;
+      v[i] = T(__builtin_inf());  // empty slots sort after every entry
+      id[i] = INT_MAX;
+    }
   }
   const size_t rbase = (size_t)heap_row(a, j) * a.ldk;
   T rv = T(0);
@@ -953,6 +1007,74 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     ri = a.I[rbase + lane];
   } else {
     rtk = true;
+  }
+
+  // Selection by sorting.  T = the k-th smallest of the 64 lane minima (k
+  // distinct entries, so T is at or above the row's final k-th entry); the
+  // entries at or below T in canonical order (few more than k while k is
+  // well below 64) go to LDS, one per lane after a wave prefix sum, and one
+  // bitonic sort over the wave orders them.  More than 64 such entries (k
+  // near 64, or many ties): the rounds below.
+  {
+    T md = __builtin_inf();
+    int mi = INT_MAX;
+    if (!rtk) {
+      md = rv;
+      mi = ri;
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      if (cless(v[i], id[i], md, mi)) {
+        md = v[i];
+        mi = id[i];
+      }
+    wave_sort_asc(md, mi, lane);
+    const T Td = __shfl(md, k - 1);
+    const int Ti = __shfl(mi, k - 1);
+    // Entries at or below T: bit i for candidate slot i, rsel for the
+    // resident entry.
+    const bool rsel = !rtk && !cless(Td, Ti, rv, ri);
+    unsigned sel = 0u;
+#pragma unroll
+    for (int i = 0; i < P; ++i) sel |= (!cless(Td, Ti, v[i], id[i]) ? 1u : 0u) << i;
+    sel &= ~tk;
+    const int cnt = __popc(sel) + (rsel ? 1 : 0);
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    const int tot = __shfl(inc, 63);
+    if (tot <= 64) {
+      // Scratch after the chunk loop's last barrier (no wave reads the tile
+      // or the query rows any more): 64 entries per wave.
+      T *sd_ = (T *)smem + w * 64;
+      int *si_ = (int *)(smem + 256 * sizeof(T)) + w * 64;
+      int pos = inc - cnt;
+      if (rsel) {
+        sd_[pos] = rv;
+        si_[pos] = ri;
+        ++pos;
+      }
+#pragma unroll
+      for (int i = 0; i < P; ++i)
+        if ((sel >> i) & 1u) {
+          sd_[pos] = v[i];
+          si_[pos] = id[i];
+          ++pos;
+        }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      T ed = lane < tot ? sd_[lane] : T(__builtin_inf());
+      int ei = lane < tot ? si_[lane] : INT_MAX;
+      wave_sort_asc(ed, ei, lane);
+      if (lane < k) {  // rows are written descending
+        a.D[rbase + k - 1 - lane] = ed;
+        a.I[rbase + k - 1 - lane] = ei;
+      }
+      return;
+    }
   }
 
   // This lane's smallest entry not yet taken: (bd, bi) at slot bs (P for the
@@ -1836,7 +1958,8 @@ hipError_t launch_small_m(const Plan &p, const KnnArgs<T> &a, hipStream_t s) {
   GSKNN_SMALL_CASE(32, 2)
 #undef GSKNN_SMALL_CASE
   if (kern == nullptr) return hipErrorNotSupported;
-  const size_t lds = ((size_t)64 * ch * a.d + 256) * sizeof(T);  // [d][C] tile + 4 query rows
+  // [d][S] tile, C norms, 4 query rows
+  const size_t lds = ((size_t)small_stride(ch) * a.d + 64 * ch + 256) * sizeof(T);
   hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
