@@ -734,3 +734,34 @@ def test_mfma_forms_tiny_candidate_sets(form, m, k):
     Dg, Ig, Dr, Ir, X2 = _mfma_case(X, (amap,), bmap, k, mode)
     assert_rows(Dg, Ig, Dr, Ir)
     assert np.all((Ig >= 0).sum(axis=1) == min(m, k))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("k,spread", [(8, 3), (16, 2), (40, 3), (16, 0)])
+def test_small_m_selection_ties(dtype, k, spread):
+    """One-wave-per-query kernel on tie-heavy inputs: coordinates drawn from
+    {-spread .. spread} (spread 0: every point the same, so every distance
+    ties and the selection takes the lowest indices), repeated candidate
+    ids, two calls into carried-over rows.  The sort-based selection and its
+    fallback (more than 64 entries at or below the threshold) both meet the
+    canonical (distance, index) order bit for bit."""
+    import torch
+    print(f"[gpu] small-m ties {np.dtype(dtype).name} k={k} spread={spread}", flush=True)
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(500 + k + spread))
+    X = g.integers(-spread, spread + 1, (3000, 6)).astype(dtype)
+    X2 = oracle.sqnorm(X).astype(dtype)
+    amaps = [g.integers(0, 3000, 700).astype(np.int32), g.integers(0, 3000, 1500).astype(np.int32)]
+    bmap = g.integers(0, 3000, 150).astype(np.int32)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D = torch.full((bmap.size, k), float("inf"), dtype=torch.from_numpy(X).dtype, device=dev)
+    I = torch.full((bmap.size, k), -1, dtype=torch.int32, device=dev)
+    run = gsknn.dknn_dev if dtype == np.float64 else gsknn.sknn_dev
+    for am in amaps:
+        run(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2, torch.from_numpy(bmap).to(dev), D, I,
+            mode=gsknn.MODE_EXACT)
+    torch.cuda.synchronize()
+    Dr, Ir = ds.empty_heap(bmap.size, k, dtype)
+    for am in amaps:
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir)
