@@ -295,9 +295,11 @@ def allnn_sweep(torch, dev, args):
     iteration: the exact VALU form on the empty rows of iteration 0, the FP32
     MFMA form after (direct once, then graph-replayed), autotuned launch
     shapes; trees are built four at a time on a second stream during the
-    sweeps.  tree_ms is the device time of a batch's build divided by its
-    trees, sweep_ms the sweep's device time, iteration_ms the wall time of one
-    step (the first step of each form includes its autotuning)."""
+    sweeps.  Both forms are autotuned, and the steady form's graph captured,
+    on the first tree before the first step (autotune_and_capture_ms).
+    tree_ms is the device time of a batch's build divided by its trees,
+    sweep_ms the sweep's device time, iteration_ms the wall time of one
+    step."""
     from rnn_amd import datasets as ds
     from rnn_amd import sweep
 
@@ -308,6 +310,10 @@ def allnn_sweep(torch, dev, args):
     X = torch.from_numpy(Xn).to(dev)
     sw = sweep.RPTreeSweep(X, 32, seed=3, leaf_lo=256, leaf_hi=4096, log=log,
                            iterations=args.iters)
+    t0 = time.perf_counter()
+    sw.tune()
+    tune_ms = (time.perf_counter() - t0) * 1e3
+    log(f"c4 autotuning and graph capture {tune_ms:.1f} ms")
     for _ in range(args.iters):
         _, _, info = sw.step()
         log(f"iteration {info['iteration']} ({info['form']}): tree {info['tree_ms']:.2f} ms, "
@@ -332,6 +338,7 @@ def allnn_sweep(torch, dev, args):
            "pairs_per_iteration": [i["pairs"] for i in t],
            "pairs_per_s": pairs / (sweep_ms * 1e-3),
            "graph_replayed_iterations": sum(1 for i in t if i["graph"]),
+           "autotune_and_capture_ms": tune_ms,
            "autotuned": tuned}
     return out
 
@@ -403,7 +410,7 @@ def north_star_objects(torch, dev, args):
                                     "bf16_max_rel_dist_err_vs_fp64_bruteforce")),
             ("c4", allnn_sweep, ("N", "iterations", "form_per_iteration", "tree_ms_per_iteration",
                                  "sweep_ms_per_iteration", "iteration_ms", "pairs_per_s",
-                                 "graph_replayed_iterations")),
+                                 "graph_replayed_iterations", "autotune_and_capture_ms")),
             ("c5", serving, ("requests", "queries", "rate_req_per_s", "offered_pairs_per_s",
                              "bf16", "fp32", "rows_equal_bf16_vs_fp32"))):
         t0 = time.perf_counter()

@@ -429,6 +429,34 @@ class RPTreeSweep:
             self.items = torch.zeros((max_items, 4), dtype=torch.int32, device=self.X.device)
             self.nitems = torch.zeros(1, dtype=torch.int32, device=self.X.device)
 
+    def tune(self):
+        """Autotune both forms' launch shapes on the first tree and capture
+        the steady form's graph, before the first step (so that no step
+        includes them).  Optional: step() tunes a form on first use."""
+        import torch
+        if self.iteration != 0:
+            return
+        if 0 not in self._trees:
+            self._build_batch(0)
+        buf, leaves, rec = self._trees[0][:3]
+        rec["e1"].synchronize()
+        if self.config is None and (self.exact_first or self.mfma is None):
+            self.config, self.tune_info = autotune(self.X, self.X2, self.k, buf, leaves,
+                                                   self.dtype, log=self.log)
+        if self.mfma is not None and self.config_mfma is None:
+            self.config_mfma, self.tune_info_mfma = autotune(
+                self.X, self.X2, self.k, buf, leaves, self.dtype, log=self.log, mfma=self.mfma)
+        self._ensure_items()
+        steady = self.mfma is not None
+        if self.use_graph and self.graph is None:
+            # The steady form's kernel has run (its tuning), so its attributes
+            # are set; capturing does not execute.
+            self.graph = torch.cuda.CUDAGraph()
+            self._graph_mfma = steady
+            with torch.cuda.graph(self.graph):
+                self._launch(steady)
+            torch.cuda.synchronize()
+
     def _launch(self, use_mfma):
         import torch
         qpb, ct, grid = self.config_mfma if use_mfma else self.config

@@ -118,12 +118,15 @@ def test_batched_trees_equal_single_trees():
 
 
 @pytest.mark.gpu
-def test_sweep_config4_parameters(tmp_path, monkeypatch):
+@pytest.mark.parametrize("pretune", [False, True])
+def test_sweep_config4_parameters(tmp_path, monkeypatch, pretune):
     """The config-4 sweep parameters (k = 32, leaves of 256..4096 points,
     d = 32, 256-cluster mixture) at N = 2^16: exact median splits of a power
     of two give leaves of 256, 512, 1024 and 2048 points (a 4096-point node
-    always lies above its stop threshold, drawn below 4096).  Two iterations
-    (a direct launch, then a graph replay), each equal bitwise to the
+    always lies above its stop threshold, drawn below 4096).  Iterations (the
+    exact form, then the MFMA form launched directly and graph-replayed; with
+    pretune both forms are tuned and the MFMA graph captured before the
+    first step, so every MFMA iteration replays), each equal bitwise to the
     composition of per-leaf oracle calls."""
     import torch
     monkeypatch.setenv("GSKNN_TUNE_CACHE", str(tmp_path / "tune.json"))
@@ -132,12 +135,17 @@ def test_sweep_config4_parameters(tmp_path, monkeypatch):
     Xn = ds.config4_points(N, d, 256, seed=3)
     X = torch.from_numpy(Xn).to(dev)
     sw = sweep.RPTreeSweep(X, k, seed=3, leaf_lo=256, leaf_hi=4096)
+    if pretune:
+        sw.tune()
+        assert sw.config is not None and sw.config_mfma is not None and sw.graph is not None
     X2 = oracle.sqnorm(Xn)
     Dr = np.full((N, k), np.inf, dtype=np.float32)
     Ir = np.full((N, k), -1, dtype=np.int32)
     biggest = 0
-    for _ in range(2):
+    for _ in range(3 if pretune else 2):
         perm, leaves, info = sw.step()
+        if pretune:
+            assert info["graph"] == (info["iteration"] > 0)
         print(f"[gpu] config-4 sweep iteration {info['iteration']}: {len(leaves)} leaves", flush=True)
         sizes = [n for _, n in leaves]
         assert min(sizes) >= 128 and max(sizes) <= 4096
