@@ -25,7 +25,7 @@
 // (times 1 + 2^-10 for the rounding of delta itself).  lower = dist~ - delta
 // and upper = dist~ + delta bracket D.
 //
-// Pass 1 (knn_bf16_filter2_kernel for 48 <= k <= 64, knn_bf16_filter_kernel
+// Pass 1 (knn_bf16_filter2_kernel for k <= 64, knn_bf16_filter_kernel
 // otherwise): per query, U = k-th smallest of {upper bounds of all
 // candidates} U {resident heap distances} (or a valid bound above it); U
 // bounds the final k-th distance from above.  Every candidate whose lower
@@ -1616,16 +1616,24 @@ Bf16Plan plan_bf16(int d, int k) {
       break;
     }
   }
-  // Per-lane heaps from k = 48 on (12 entries or more per lane): at config 3
-  // (k = 64) the filter runs 23.2 ms against 29.0 with one heap per query,
-  // with 28 % longer lists; for small k the largest of four short-heap roots
-  // is far above the exact U and the lists double.  Their lists get twice
-  // the capacity: an overflowing list is re-scanned against every candidate.
-  p.pl = k >= 48;
+  // Per-lane upper bounds (the register form for k <= 4 kRegRK, the LDS form's
+  // per-lane heaps from k = 48 on) get twice the list capacity: an
+  // overflowing list is re-scanned against every candidate.  In the LDS form
+  // the largest of four short-heap roots is far above the exact U for small
+  // k (one heap per query there); the register form's warm-up pass sets U
+  // from each lane's tile minima first, and at config-3 size it runs k = 10,
+  // 16, 24, 32 in 12.3, 13.0, 13.3, 14.0 ms against 19.4, 22.2, 25.7, 28.6 ms
+  // for the LDS form, with shorter lists.  GSKNN_BF16_REG_MINK=<k> keeps the
+  // LDS form below k (tuning).
+  static const int reg_mink = [] {
+    const char *e = getenv("GSKNN_BF16_REG_MINK");
+    return e ? atoi(e) : 1;
+  }();
+  p.pl = k >= 48 || (k >= reg_mink && k <= 4 * kRegRK);
   const int cap = (p.pl ? 32 : 16) * k;
   const int lo = p.pl ? 1024 : 512, hi = p.pl ? 8192 : 4096;
   p.cap = cap < lo ? lo : (cap > hi ? hi : cap);
-  // Register form for 48 <= k <= 4 kRegRK (two queries blocks, four waves;
+  // Register form for k <= 4 kRegRK (two queries blocks, four waves;
   // GSKNN_BF16_REG=0 keeps the LDS-heap form, for comparison).
   static const bool reg_on = [] {
     const char *e = getenv("GSKNN_BF16_REG");

@@ -1,8 +1,8 @@
 """The re-scan fallback of the two-pass forms: when a query's candidate list
 (in one candidate split) outgrows its capacity (clamp(16 k, 512, 4096)
-entries; for the BF16 form with per-lane upper-bound heaps, k >= 48,
-clamp(32 k, 1024, 8192)), the re-rank re-scans every candidate of the split
-exactly.
+entries for FP32; for the BF16 form's per-lane upper bounds, k <= 64 or
+k >= 48, clamp(32 k, 1024, 8192)), the re-rank re-scans every candidate of
+the split exactly.
 Thousands of candidates at exactly the same distance from every query
 (duplicated rows with distinct indices) all reach the k-th boundary; with
 16384 queries a launch splits the candidates at most a few ways, so every
@@ -64,10 +64,11 @@ def test_fp32_list_overflow_rescans_exactly(d):
     assert nover >= n, f"{nover} (split, query) lists overflowed for {n} queries"
 
 
-@pytest.mark.parametrize("d,k,copies", [(32, 10, 3000), (128, 10, 3000), (64, 64, 12000)])
+@pytest.mark.parametrize("d,k,copies", [(32, 10, 12000), (128, 10, 12000), (64, 64, 12000)])
 def test_bf16_list_overflow_rescans_exactly(d, k, copies):
-    # k = 64 takes the per-lane heaps (list capacity 2048): 12000 tied
-    # copies give each of the few candidate splits about 3000.
+    # The register form (k <= 64) lists in four lane-group segments of
+    # capacity clamp(32 k, 1024, 8192) / 4: 12000 tied copies give each of
+    # the few candidate splits about 3000, so every segment overflows.
     import torch
     print(f"[gpu] bf16 overflow d={d} k={k}", flush=True)
     X, amap, bmap, rows = _tied_table(d, np.float32, copies)
