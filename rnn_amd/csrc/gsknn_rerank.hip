@@ -1557,7 +1557,11 @@ namespace {
 constexpr size_t kLds = 160 * 1024;
 constexpr int kCBb = 4;
 
-constexpr int kRegRK = 16;  // register form: upper-bound set size per lane (k <= 4 kRegRK)
+// Register form: upper-bound set size per lane, two query blocks per wave
+// for k <= 4 kRegRK, one query block (fewer query registers, larger sets)
+// for 4 kRegRK < k <= 4 kRegRK1.
+constexpr int kRegRK = 16;
+constexpr int kRegRK1 = 32;
 
 struct Bf16Plan {
   int ks = 0, qb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
@@ -1629,7 +1633,7 @@ Bf16Plan plan_bf16(int d, int k) {
     const char *e = getenv("GSKNN_BF16_REG_MINK");
     return e ? atoi(e) : 1;
   }();
-  p.pl = k >= 48 || (k >= reg_mink && k <= 4 * kRegRK);
+  p.pl = k >= 48 || (k >= reg_mink && k <= 4 * kRegRK1);
   const int cap = (p.pl ? 32 : 16) * k;
   const int lo = p.pl ? 1024 : 512, hi = p.pl ? 8192 : 4096;
   p.cap = cap < lo ? lo : (cap > hi ? hi : cap);
@@ -1639,11 +1643,16 @@ Bf16Plan plan_bf16(int d, int k) {
     const char *e = getenv("GSKNN_BF16_REG");
     return !(e && atoi(e) == 0);
   }();
-  if (reg_on && p.pl && k <= 4 * kRegRK && Filter2Layout(p.ks, kCBb).bytes <= kLds) {
+  static const int reg1_on = [] {
+    const char *e = getenv("GSKNN_BF16_REG1");
+    return !(e && atoi(e) == 0);
+  }();
+  if (reg_on && p.pl && (k <= 4 * kRegRK || (reg1_on && k <= 4 * kRegRK1)) &&
+      Filter2Layout(p.ks, kCBb).bytes <= kLds) {
     p.reg = true;
-    p.qb = 2;
+    p.qb = k <= 4 * kRegRK ? 2 : 1;
     p.w = 4;
-    p.q = 4 * 16 * 2;
+    p.q = 4 * 16 * p.qb;
     p.lds = Filter2Layout(p.ks, kCBb).bytes;
     p.G = 4;
   }
@@ -1690,7 +1699,8 @@ template <int KS>
 hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s) {
   const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
   if (p.reg) {
-    auto kern = knn_bf16_filter2_kernel<KS, 2, kCBb, 4, kRegRK>;
+    auto kern = p.qb == 2 ? knn_bf16_filter2_kernel<KS, 2, kCBb, 4, kRegRK>
+                          : knn_bf16_filter2_kernel<KS, 1, kCBb, 4, kRegRK1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)p.lds);
     if (e != hipSuccess) return e;

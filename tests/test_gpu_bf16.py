@@ -1,5 +1,5 @@
 """GPU parity of the BF16 candidate pass + FP64 re-rank in its register form
-(k <= 64: upper-bound sets in registers, one list segment per lane
+(k <= 128: upper-bound sets in registers, one list segment per lane
 group, candidates gathered into a panel) against the FP64 oracle.
 
 Bar: neighbour indices exact; distances within
@@ -50,7 +50,8 @@ def _oracle(X, amaps, bmap, k, canonical=False):
 
 
 @pytest.mark.parametrize("d,k", [(256, 48), (256, 64), (97, 50), (50, 64), (8, 56), (200, 63),
-                                 (64, 1), (100, 3), (256, 7), (33, 17), (128, 32), (256, 10)])
+                                 (64, 1), (100, 3), (256, 7), (33, 17), (128, 32), (256, 10),
+                                 (256, 65), (128, 100), (70, 128)])
 def test_register_form_two_calls(d, k):
     # Low-rank points, two calls of disjoint candidates into the same rows.
     g = np.random.Generator(np.random.PCG64(1000 + d + k))
