@@ -443,13 +443,19 @@ struct RingCfg {
 // query blocks per wave (each candidate operand feeds two MFMAs) and four
 // candidate blocks per tile up to d = 112 (the per-tile epilogue amortised
 // over 16 MFMAs per pair), two up to d = 192 and one up to d = 256, where
-// the query operands take more registers.
+// the query operands take more registers; for d in 257 .. 320 one query
+// block per wave (its 40 pairs of operands in the registers two blocks of 32
+// take): d = 300 runs 55.2 ms against 81.7 for the streamed form, odd d = 283
+// (8-byte loads) 101 against 109 ms.  Measured and left out: 48 pairs with
+// one candidate block (d = 352 / 384: 88.9 / 89.2 ms, no faster than the
+// streamed form); 56 and more pairs spill.
 constexpr int kRingCfg[][4] = {{2, 2, 4, 1},   {4, 2, 4, 1},   {6, 2, 4, 1},   {8, 2, 4, 1},
                                {10, 2, 4, 1},  {12, 2, 4, 1},  {14, 2, 4, 1},  {16, 2, 2, 1},
-                               {20, 2, 2, 1},  {24, 2, 2, 1},  {28, 2, 1, 1},  {32, 2, 1, 1}};
-constexpr int kNumRingCfg = 12;
+                               {20, 2, 2, 1},  {24, 2, 2, 1},  {28, 2, 1, 1},  {32, 2, 1, 1},
+                               {40, 1, 2, 1}};
+constexpr int kNumRingCfg = 13;
 
-// Above d = 256: queries streamed (KP = 0), 2 x 2 blocks.
+// Above d = 256 otherwise: queries streamed (KP = 0), 2 x 2 blocks.
 constexpr int kRingStreamCfg[4] = {0, 2, 2, 1};
 
 bool ring_cfg(int d, RingCfg *c) {
@@ -494,6 +500,7 @@ int ring_reg_blocks_t() {
   X(24, 2, 2, 1)             \
   X(28, 2, 1, 1)             \
   X(32, 2, 1, 1)             \
+  X(40, 1, 2, 1)             \
   X(0, 2, 2, 1)
 
 }  // namespace

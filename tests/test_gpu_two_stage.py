@@ -123,7 +123,7 @@ def _lattice_case(d):
     _assert_bitwise(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir)
 
 
-@pytest.mark.parametrize("d", [96, 128, 256, 300])
+@pytest.mark.parametrize("d", [96, 128, 256, 258, 283, 300, 320])
 @pytest.mark.parametrize("warm", [False, True])
 def test_two_stage_random_rows_match_oracle(d, warm):
     _random_case(d, 32768, warm)
