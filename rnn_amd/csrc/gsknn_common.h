@@ -22,6 +22,22 @@ __device__ __forceinline__ float f32_pass_thr(float U, float nq, float dmx) {
   return __builtin_fmaf(0x1p-18f, fabsf(U) + fabsf(nq) + dmx, (U - nq) + dmx);
 }
 
+// v[bb] for a per-lane bb in [0, N): a tree of selects on the bits of bb
+// (N - 1 selects).  The values must be scalars (FP32 results): a select
+// between two elements of one vector would become an element at a run-time
+// index, a chain of compares per level.
+template <int LO, int N, int NV>
+__device__ __forceinline__ float select_tree(const float (&v)[NV], int bb) {
+  if constexpr (N == 1) {
+    return v[LO];
+  } else {
+    constexpr int H = N / 2;
+    const float lo = select_tree<LO, H, NV>(v, bb);
+    const float hi = select_tree<LO + H, H, NV>(v, bb);
+    return (bb & H) ? hi : lo;
+  }
+}
+
 // Kernel arguments of one kNN launch.  All pointers are device pointers.
 template <typename T>
 struct KnnArgs {

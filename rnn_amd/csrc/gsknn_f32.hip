@@ -104,6 +104,9 @@ struct F32Args {
   int *cnt;     // [nsplit][n]
   float *U;     // [nsplit][n]
   int *nover;
+  int G;                       // list segments per (split, query): cnt is [nsplit][G][n]
+  const unsigned char *panel;  // register form: candidates in position order
+  int warm;                    // register form: tiles of each split in the warm-up pass
 };
 
 // Norms of the gathered points (FP64 sum, rounded up) and candidate maxima.
@@ -467,6 +470,322 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register form of pass 1 (d <= 128, k <= 64), the structure of the BF16
+// register-form filter on FP32 operands: candidates gathered into a panel in
+// position order (FP32 rows zero padded to DP = 16 NB dims and a record
+// {XA2, |a| rounded up} per position), tiles of 64 candidates loaded straight
+// into LDS (global_load_lds_dwordx4) into two buffers with one barrier per
+// tile, per-lane sets of the k/4 smallest upper bounds in registers, one list
+// segment per lane group, a warm-up pass over the first tiles.
+//
+// Dims are dealt in 16-dim blocks: in block j, k-step i of lane group g
+// (g = l / 16) carries dim 16 j + 4 g + i, so one 16-byte load (and one
+// ds_read_b128) per lane and candidate row gives the operands of four
+// k-steps; the query operands follow the same deal.  Which dims a k-step
+// carries is free as long as both operands agree: the MFMA sum is bounded in
+// any order.
+//
+// Upper bounds enter the sets clamped at 0: an FP32 oracle distance can be
+// slightly negative (no clamp), and the sets order FP32 bit patterns as
+// integers, which orders negative values wrongly; a larger bound stays valid.
+// ---------------------------------------------------------------------------
+constexpr int kF32Pad = 64;
+struct F32Panel {
+  size_t rows, rec, bytes;
+  __host__ __device__ F32Panel(int m, int DP) {
+    rows = 0;
+    rec = a16((size_t)(m + kF32Pad) * DP * 4);
+    bytes = a16(rec + (size_t)(m + kF32Pad) * 16);
+  }
+};
+
+// One wave per candidate position (pad positions written as zeros).
+__global__ __launch_bounds__(256) void f32_gather_kernel(F32Args a, int DP, unsigned char *panel) {
+  const F32Panel P(a.m, DP);
+  const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (pos >= a.m + kF32Pad) return;
+  float *dst = (float *)(panel + P.rows) + (size_t)pos * DP;
+  if (pos >= a.m) {
+    for (int c = lane; c < DP; c += 64) dst[c] = 0.0f;
+    if (lane == 0) ((float4 *)(panel + P.rec))[pos] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const int id = a.amap[pos];
+  const float *src = a.XA + (size_t)id * a.d;
+  for (int c = lane; c < DP; c += 64) dst[c] = c < a.d ? src[c] : 0.0f;
+  if (lane == 0) ((float4 *)(panel + P.rec))[pos] = make_float4(a.XA2[id], a.xnA[pos], 0.f, 0.f);
+}
+
+struct F32RegLayout {
+  size_t tile, nc, buf, bytes;
+  __host__ __device__ F32RegLayout(int NB, int CB) {
+    tile = 0;
+    nc = a16(tile + (size_t)NB * CB * 1024);
+    buf = a16(nc + 16 * (size_t)CB * 4);
+    bytes = 2 * buf;
+  }
+};
+
+template <int NB, int QB, int CB, int W, int RK>
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(2))) void knn_f32_filter2_kernel(F32Args a) {
+  constexpr int QW = 16 * QB, Q = W * QW, C = 16 * CB;
+  constexpr int DP = 16 * NB;
+  constexpr int BLK = NB * CB;  // 64 x 16-byte fragment blocks per tile
+  static_assert(BLK % W == 0, "fragment blocks per wave");
+  static_assert(C == 64, "one candidate per lane");
+  constexpr int BPW = BLK / W;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const F32RegLayout L(NB, CB);
+  const int k = a.k, d = a.d;
+  const F32Panel PL(a.m, DP);
+  const float *xg = (const float *)(a.panel + PL.rows);
+  const float4 *grec = (const float4 *)(a.panel + PL.rec);
+
+  const int tid = threadIdx.x, lane = tid & 63, grp = lane >> 4, si = lane & 15;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int qbase = blockIdx.x * Q;
+  const int split = blockIdx.y;
+  const int cbeg = split * a.cps;
+  const int cend = min(a.m, cbeg + a.cps);
+  const int capg = a.cap / 4;
+
+  // Bound of the FP32 MFMA value against the oracle's FP32 distance (the
+  // header of this file): delta = cd |a||b| + 4u (|nc| + |nq|).
+  const float u = kU;
+  const double gm = (2.0 * d + 2) * 2.0 * 0x1p-24 / (1.0 - (2.0 * d + 2) * 2.0 * 0x1p-24);
+  const double go = d * 0x1p-24 / (1.0 - d * 0x1p-24);
+  const float cd = (float)((2.0 * gm + 2.0 * go + 8.1 * 0x1p-24) * 1.01);
+  float4 bq[QB][NB];
+  float nq[QB], al[QB], cq[QB], dmx[QB];
+  int ql[QB];
+  bool qv[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    ql[qb] = w * QW + qb * 16 + si;
+    const int j = qbase + ql[qb];
+    qv[qb] = j < a.n;
+    const int jj = qv[qb] ? j : a.n - 1;
+    const float *xq = a.XB + (size_t)a.bmap[jj] * d;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = 16 * b + 4 * grp + i;
+        v[i] = (qv[qb] && p < d) ? xq[p] : 0.0f;
+      }
+      bq[qb][b] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    nq[qb] = a.XB2[a.bmap[jj]];
+    al[qb] = cd * a.xnB[jj];
+    cq[qb] = 4.0f * u * fabsf(nq[qb]);
+    dmx[qb] = (al[qb] * a.mx[0] + 4.0f * u * a.mx[1] + cq[qb]) * 1.002f + 1e-30f;
+  }
+
+  const int kg = (k + 3 - grp) / 4;
+  int hu[QB][RK];
+  auto insert_bits = [&](int qb, int x) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = RK - 1; i >= 1; --i) hu[qb][i] = max(hu[qb][i - 1], min(hu[qb][i], x));
+    hu[qb][0] = min(hu[qb][0], x);
+  };
+  constexpr int kNegInf = (int)0xFF800000u, kPosInf = 0x7F800000;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+    for (int i = 0; i < RK; ++i) hu[qb][i] = i < RK - kg ? kNegInf : kPosInf;
+    if (qv[qb]) {
+      const int j = qbase + ql[qb];
+      const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+      for (int e = 0; e < kg; ++e) insert_bits(qb, max(__float_as_int(a.D[row + grp + 4 * e]), 0));
+    }
+  }
+  auto group_u = [&](int qb) -> float {
+    float r = __int_as_float(hu[qb][RK - 1]);
+    r = fmaxf(r, __shfl_xor(r, 16));
+    r = fmaxf(r, __shfl_xor(r, 32));
+    return r;
+  };
+  float uq[QB], thr[QB];
+  auto refresh = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      uq[qb] = group_u(qb);
+      thr[qb] = qv[qb] ? f32_pass_thr(uq[qb], nq[qb], dmx[qb]) : -__builtin_inff();
+    }
+  };
+  refresh();
+  int lc[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) lc[qb] = 0;
+
+  // Tile staging: wave w loads fragment blocks w BPW .. w BPW + BPW - 1
+  // (dim block bidx / CB, candidate block bidx % CB); lane l reads dims
+  // 16 j + 4 (l / 16) .. + 4 of candidate (bidx % CB) 16 + l % 16.
+  auto tile_rows = [&](int t) { return (const char *)xg + (size_t)(cbeg + t * C) * DP * 4; };
+  auto tile_recs = [&](int t) { return grec + cbeg + t * C; };
+  unsigned doff[BPW];
+#pragma unroll
+  for (int e = 0; e < BPW; ++e) {
+    const int bidx = w * BPW + e;
+    doff[e] = (unsigned)(((bidx % CB) * 16 + si) * DP * 4 + ((bidx / CB) * 16 + 4 * grp) * 4);
+  }
+  const int ntiles = cend > cbeg ? (cend - cbeg + C - 1) / C : 0;
+  const int T0 = min(a.warm, ntiles);
+  const int nsteps = ntiles > 0 ? T0 + ntiles : 0;
+  auto tile_of = [&](int v) { return v < T0 ? v : v - T0; };
+  auto bufp = [&](int v) -> unsigned char * { return smem + (size_t)(v & 1) * L.buf; };
+  float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
+  float rc_x = 0.f, rc_n = 0.f;
+  auto issue_tile = [&](int v) __attribute__((always_inline)) {
+    unsigned char *b = bufp(v);
+    const int t = tile_of(v);
+    const char *rows = tile_rows(t);
+#pragma unroll
+    for (int e = 0; e < BPW; ++e) {
+      const int bidx = w * BPW + e;
+      __builtin_amdgcn_global_load_lds((const void *)(rows + doff[e]),
+                                       (__attribute__((address_space(3))) void *)(b + L.tile + (size_t)bidx * 1024),
+                                       16, 0, 0);
+    }
+    st = tile_recs(t)[lane];
+  };
+  auto finish_tile = [&](int v) __attribute__((always_inline)) {
+    unsigned char *b = bufp(v);
+    const bool ok = cbeg + tile_of(v) * C + lane < cend;
+    rc_n = ok ? st.x : __builtin_inff();
+    rc_x = ok ? st.y : 0.0f;
+    if (grp == w) ((float *)(b + L.nc))[lane] = rc_n;
+  };
+  auto wait_loads = []() __attribute__((always_inline)) { __builtin_amdgcn_s_waitcnt(0x0F70); };
+
+  if (nsteps > 0) {
+    issue_tile(0);
+    wait_loads();
+    finish_tile(0);
+  }
+
+  for (int step = 0; step < nsteps; ++step) {
+    __syncthreads();  // this step's tile in LDS; every wave done with the last step
+    const int t = tile_of(step);
+    const bool more = step + 1 < nsteps;
+    if (more) issue_tile(step + 1);
+    const unsigned char *b = bufp(step);
+    const float4 *tA = (const float4 *)(b + L.tile);
+    const float *tnc = (const float *)(b + L.nc);
+    f32x4 acc[QB][CB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[qb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int jb = 0; jb < NB; ++jb) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const float4 av = tA[(jb * CB + cb) * 64 + lane];
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bq[qb][jb].x, acc[qb][cb], 0, 0, 0);
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bq[qb][jb].y, acc[qb][cb], 0, 0, 0);
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bq[qb][jb].z, acc[qb][cb], 0, 0, 0);
+          acc[qb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bq[qb][jb].w, acc[qb][cb], 0, 0, 0);
+        }
+      }
+    }
+    float tq[QB][CB * 4];
+    float tmin[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      tmin[qb] = __builtin_inff();
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = cb * 16 + 4 * grp + v;
+          const float tt = __builtin_fmaf(-2.0f, acc[qb][cb][v], tnc[c]);
+          tq[qb][cb * 4 + v] = tt;
+          tmin[qb] = fminf(tmin[qb], tt);
+        }
+    }
+    if (step < T0) {
+      // Warm-up step: each lane's smallest tt of the tile bounds one
+      // candidate's oracle distance from above by (tmin + nq) + dmx.
+      int xb[QB];
+      bool need = false;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        const float s = (tmin[qb] + nq[qb]) + dmx[qb];
+        xb[qb] = qv[qb] ? max(__float_as_int(widen_up(s)), 0) : kPosInf;
+        need |= xb[qb] < hu[qb][RK - 1];
+      }
+      if (__any(need)) {
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) insert_bits(qb, xb[qb]);
+        refresh();
+      }
+    }
+    // Main pass; over the warm-up tiles survivors are listed only.
+    const bool ins = step >= 2 * T0;
+    bool hit = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) hit |= step >= T0 && tmin[qb] <= thr[qb];
+    if (__any(hit)) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        unsigned bits = 0u;
+        if (tmin[qb] <= thr[qb]) {
+#pragma unroll
+          for (int bb = 0; bb < CB * 4; ++bb) bits |= (tq[qb][bb] <= thr[qb] ? 1u : 0u) << bb;
+        }
+        const int j = qbase + ql[qb];
+        const size_t lb = ((size_t)split * a.n + j) * a.cap + (size_t)grp * capg;
+        // Wave-uniform loop (the record shuffles need every lane active);
+        // a lane without a survivor inserts +inf, which changes nothing.
+        if (__any(bits != 0u)) do {
+          const bool act = bits != 0u;
+          const int bb = act ? __builtin_ctz(bits) : 0;
+          bits &= bits - 1;
+          const float tt = select_tree<0, CB * 4, CB * 4>(tq[qb], bb);
+          const int c = (bb >> 2) * 16 + 4 * grp + (bb & 3);
+          const float rx = __shfl(rc_x, c), rn = __shfl(rc_n, c);
+          const int pos = cbeg + t * C + c;
+          const bool valid = act && pos < cend;
+          const float dist = tt + nq[qb];
+          const float delta = rx * al[qb] + 4.0f * u * fabsf(rn) + cq[qb] + 1e-30f;
+          const float lower = widen_dn(dist - delta), upper = widen_up(dist + delta);
+          const bool app = valid && lower <= uq[qb];
+          const int at = lc[qb];
+          lc[qb] = at + (app ? 1 : 0);
+          if (app && at < capg) a.lst[lb + at] = make_int2(pos, __float_as_int(lower));
+          const int ux = max(__float_as_int(upper), valid && ins ? 0 : kPosInf);
+          if (__any(ux < hu[qb][RK - 1])) insert_bits(qb, ux);
+        } while (__any(bits != 0u));
+      }
+      refresh();
+    }
+    if (more) {
+      wait_loads();
+      finish_tile(step + 1);
+    }
+  }
+
+  float uf[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) uf[qb] = group_u(qb);
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int j = qbase + ql[qb];
+    if (j < a.n) {
+      a.cnt[((size_t)split * 4 + grp) * a.n + j] = lc[qb];
+      if (grp == 0) a.U[(size_t)split * a.n + j] = uf[qb];
+      if (lc[qb] > capg) atomicAdd(a.nover, 1);
+    }
+  }
+}
+
 // The oracle's FP32 distance for candidate position pos and query j.
 __device__ __forceinline__ float oracle_dist_f32(const F32Args &a, int pos, int j) {
   const float *xa = a.XA + (size_t)a.amap[pos] * a.d;
@@ -501,12 +820,16 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   float Ulim = a.U[j];
   for (int sp = 1; sp < a.nsplit; ++sp) Ulim = fminf(Ulim, a.U[(size_t)sp * a.n + j]);
+  const int capg = a.cap / a.G;
   for (int sp = 0; sp < a.nsplit; ++sp) {
     const size_t sj = (size_t)sp * a.n + j;
-    const int cnt = a.cnt[sj];
-    const bool over = cnt > a.cap;
+    bool over = false;
+    for (int gg = 0; gg < a.G; ++gg) over |= a.cnt[((size_t)sp * a.G + gg) * a.n + j] > capg;
     const int cbeg = sp * a.cps;
-    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : cnt;
+    // An overflowed split is re-scanned once; otherwise each of its lists.
+    for (int gg = 0; gg < (over ? 1 : a.G); ++gg) {
+    const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : a.cnt[((size_t)sp * a.G + gg) * a.n + j];
+    const size_t lbase = sj * a.cap + (size_t)gg * capg;
     for (int e0 = 0; e0 < nscan; e0 += 64) {
       const int e = e0 + lane;
       bool have = false;
@@ -516,7 +839,7 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
           pos = cbeg + e;
           have = true;
         } else {
-          const int2 en = a.lst[sj * a.cap + e];
+          const int2 en = a.lst[lbase + e];
           pos = en.x;
           have = __int_as_float(en.y) <= Ulim;
         }
@@ -539,6 +862,7 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
     }
+    }
   }
   if (lane == 0) {
     for (int e2 = k - 1; e2 > 0; --e2) {
@@ -558,10 +882,14 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
 namespace {
 
 constexpr size_t kLdsF = 160 * 1024;
+constexpr int kF32RegRK = 16;  // register form: upper-bound set size per lane
 
 struct F32Plan {
   int ks = 0, qb = 0, cb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
   size_t lds = 0;
+  bool reg = false;  // register form (knn_f32_filter2_kernel)
+  int nb = 0;        // register form: 16-dim blocks (DP = 16 nb)
+  int G = 1;         // list segments per (split, query)
 };
 
 // Split the candidates when the query blocks alone give fewer than two
@@ -612,21 +940,41 @@ F32Plan plan_f32(int d, int k) {
   }
   const int cap = 16 * k;
   p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
+  // Register form for d <= 128 and k <= 4 kF32RegRK (GSKNN_F32_REG=0 keeps
+  // the LDS-tile form, for comparison); its lists get twice the capacity,
+  // in four lane-group segments.
+  static const bool reg_on = [] {
+    const char *e = getenv("GSKNN_F32_REG");
+    return !(e && atoi(e) == 0);
+  }();
+  if (reg_on && d <= 128 && k <= 4 * kF32RegRK) {
+    p.reg = true;
+    p.nb = d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 4 : 8;
+    p.qb = 2;
+    p.cb = 4;
+    p.w = 4;
+    p.q = 4 * 16 * 2;
+    p.lds = F32RegLayout(p.nb, 4).bytes;
+    p.G = 4;
+    const int c2 = 32 * k;
+    p.cap = c2 < 1024 ? 1024 : (c2 > 8192 ? 8192 : c2);
+  }
   return p;
 }
 
 struct F32Work {
-  size_t xnA, xnB, mx, lst, cnt, U, nover, bytes;
-  F32Work(int m, int n, int cap, int nsplit) {
-    const size_t rows = (size_t)n * nsplit;
+  size_t xnA, xnB, mx, lst, cnt, U, nover, panel, bytes;
+  F32Work(int m, int n, const F32Plan &p) {
+    const size_t rows = (size_t)n * p.nsplit;
     xnA = 0;
     xnB = a16(xnA + (size_t)m * 4);
     mx = a16(xnB + (size_t)n * 4);
     lst = a16(mx + 16);
-    cnt = a16(lst + rows * cap * 8);
-    U = a16(cnt + rows * 4);
+    cnt = a16(lst + rows * p.cap * 8);
+    U = a16(cnt + rows * p.G * 4);
     nover = a16(U + rows * 4);
-    bytes = a16(nover + 16);
+    panel = a16(nover + 16);
+    bytes = a16(panel + (p.reg ? F32Panel(m, 16 * p.nb).bytes : 0));
   }
 };
 
@@ -659,14 +1007,14 @@ size_t f32_work_bytes(int m, int n, int d, int k) {
   F32Plan p = plan_f32(d, k);
   if (p.q == 0 || m <= 0 || n <= 0) return 0;
   plan_f32_splits(p, m, n);
-  return F32Work(m, n, p.cap, p.nsplit).bytes;
+  return F32Work(m, n, p).bytes;
 }
 
 size_t f32_overflow_offset(int m, int n, int d, int k) {
   F32Plan p = plan_f32(d, k);
   if (p.q == 0 || m <= 0 || n <= 0) return (size_t)-1;
   plan_f32_splits(p, m, n);
-  return F32Work(m, n, p.cap, p.nsplit).nover;
+  return F32Work(m, n, p).nover;
 }
 
 bool f32_mfma_supported(int d, int k) {
@@ -682,7 +1030,7 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
   plan_f32_splits(p, ka.m, ka.n);
   const size_t rl = a16(sizeof(float) * 4 * (size_t)ka.k) + sizeof(int) * 4 * (size_t)ka.k;
   if (rl > kLdsF) return hipErrorNotSupported;
-  const F32Work W(ka.m, ka.n, p.cap, p.nsplit);
+  const F32Work W(ka.m, ka.n, p);
   if (work == nullptr || work_bytes < W.bytes) return hipErrorInvalidValue;
   char *wb = (char *)work;
   F32Args a{};
@@ -698,11 +1046,33 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
   a.cnt = (int *)(wb + W.cnt);
   a.U = (float *)(wb + W.U);
   a.nover = (int *)(wb + W.nover);
+  a.G = p.G;
+  a.panel = (const unsigned char *)(wb + W.panel);
+  // Warm-up pass of the register form: the first 1/16 of each split's tiles
+  // (GSKNN_F32_WARM=<tiles>, read at each call, sets the count).
+  const char *warm_e = getenv("GSKNN_F32_WARM");
+  a.warm = p.reg ? (warm_e ? atoi(warm_e) : p.cps / 64 / 16) : 0;
   hipLaunchKernelGGL(f32_zero_kernel, dim3(1), dim3(64), 0, s, a.mx, a.nover);
   hipLaunchKernelGGL(f32_prep_kernel, dim3((a.m + a.n + 255) / 256), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  switch (p.ks) {
+  if (p.reg) {
+    hipLaunchKernelGGL(f32_gather_kernel, dim3((a.m + kF32Pad + 3) / 4), dim3(256), 0, s, a, 16 * p.nb,
+                       (unsigned char *)(wb + W.panel));
+    const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
+    void (*kern)(F32Args) = nullptr;
+    switch (p.nb) {
+      case 1: kern = knn_f32_filter2_kernel<1, 2, 4, 4, kF32RegRK>; break;
+      case 2: kern = knn_f32_filter2_kernel<2, 2, 4, 4, kF32RegRK>; break;
+      case 4: kern = knn_f32_filter2_kernel<4, 2, 4, 4, kF32RegRK>; break;
+      case 8: kern = knn_f32_filter2_kernel<8, 2, 4, 4, kF32RegRK>; break;
+      default: return hipErrorNotSupported;
+    }
+    e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(256), p.lds, s, a);
+    e = hipGetLastError();
+  } else switch (p.ks) {
     case 4: e = launch_f32_ks<4>(p, a, s); break;
     case 8: e = launch_f32_ks<8>(p, a, s); break;
     case 16: e = launch_f32_ks<16>(p, a, s); break;

@@ -172,21 +172,6 @@ __device__ __forceinline__ size_t list_base(const Bf16Args &a, int sp, int gg, i
 // is row kRowOf(l, v), column l%16.
 __device__ __forceinline__ int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
 
-// v[bb] for a per-lane bb in [0, N): a tree of selects on the bits of bb
-// (N - 1 selects).  The values must be scalars (FP32 results): a select
-// between two elements of one vector would become an element at a run-time
-// index, a chain of compares per level.
-template <int LO, int N, int NV>
-__device__ __forceinline__ float select_tree(const float (&v)[NV], int bb) {
-  if constexpr (N == 1) {
-    return v[LO];
-  } else {
-    constexpr int H = N / 2;
-    const float lo = select_tree<LO, H, NV>(v, bb);
-    const float hi = select_tree<LO + H, H, NV>(v, bb);
-    return (bb & H) ? hi : lo;
-  }
-}
 
 struct FilterLayout {
   size_t tA, nc, xn, en, id, heap, cnt, bytes;
