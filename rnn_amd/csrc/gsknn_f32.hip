@@ -882,7 +882,10 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
 namespace {
 
 constexpr size_t kLdsF = 160 * 1024;
-constexpr int kF32RegRK = 16;  // register form: upper-bound set size per lane
+// Register form: upper-bound set size per lane, two query blocks per wave
+// for k <= 4 kF32RegRK, one (larger sets) for 4 kF32RegRK < k <= 4 kF32RegRK1.
+constexpr int kF32RegRK = 16;
+constexpr int kF32RegRK1 = 32;
 
 struct F32Plan {
   int ks = 0, qb = 0, cb = 0, w = 0, q = 0, cap = 0, nsplit = 1, cps = 0;
@@ -940,20 +943,20 @@ F32Plan plan_f32(int d, int k) {
   }
   const int cap = 16 * k;
   p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
-  // Register form for d <= 128 and k <= 4 kF32RegRK (GSKNN_F32_REG=0 keeps
+  // Register form for d <= 128 and k <= 4 kF32RegRK1 (GSKNN_F32_REG=0 keeps
   // the LDS-tile form, for comparison); its lists get twice the capacity,
   // in four lane-group segments.
   static const bool reg_on = [] {
     const char *e = getenv("GSKNN_F32_REG");
     return !(e && atoi(e) == 0);
   }();
-  if (reg_on && d <= 128 && k <= 4 * kF32RegRK) {
+  if (reg_on && d <= 128 && k <= 4 * kF32RegRK1) {
     p.reg = true;
     p.nb = d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 4 : 8;
-    p.qb = 2;
+    p.qb = k <= 4 * kF32RegRK ? 2 : 1;
     p.cb = 4;
     p.w = 4;
-    p.q = 4 * 16 * 2;
+    p.q = 4 * 16 * p.qb;
     p.lds = F32RegLayout(p.nb, 4).bytes;
     p.G = 4;
     const int c2 = 32 * k;
@@ -1061,11 +1064,15 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
                        (unsigned char *)(wb + W.panel));
     const dim3 grid((a.n + p.q - 1) / p.q, a.nsplit);
     void (*kern)(F32Args) = nullptr;
-    switch (p.nb) {
-      case 1: kern = knn_f32_filter2_kernel<1, 2, 4, 4, kF32RegRK>; break;
-      case 2: kern = knn_f32_filter2_kernel<2, 2, 4, 4, kF32RegRK>; break;
-      case 4: kern = knn_f32_filter2_kernel<4, 2, 4, 4, kF32RegRK>; break;
-      case 8: kern = knn_f32_filter2_kernel<8, 2, 4, 4, kF32RegRK>; break;
+    switch (p.nb * 4 + p.qb) {
+      case 1 * 4 + 2: kern = knn_f32_filter2_kernel<1, 2, 4, 4, kF32RegRK>; break;
+      case 2 * 4 + 2: kern = knn_f32_filter2_kernel<2, 2, 4, 4, kF32RegRK>; break;
+      case 4 * 4 + 2: kern = knn_f32_filter2_kernel<4, 2, 4, 4, kF32RegRK>; break;
+      case 8 * 4 + 2: kern = knn_f32_filter2_kernel<8, 2, 4, 4, kF32RegRK>; break;
+      case 1 * 4 + 1: kern = knn_f32_filter2_kernel<1, 1, 4, 4, kF32RegRK1>; break;
+      case 2 * 4 + 1: kern = knn_f32_filter2_kernel<2, 1, 4, 4, kF32RegRK1>; break;
+      case 4 * 4 + 1: kern = knn_f32_filter2_kernel<4, 1, 4, 4, kF32RegRK1>; break;
+      case 8 * 4 + 1: kern = knn_f32_filter2_kernel<8, 1, 4, 4, kF32RegRK1>; break;
       default: return hipErrorNotSupported;
     }
     e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);

@@ -73,6 +73,7 @@ class KnnServer:
         self.streams = [torch.cuda.Stream(device=dev) for _ in range(streams)]
         self.busy = {}
         self.reserved = 0
+        self._fixed = {}
         self.batches_launched = 0
         self.batch_sizes = []
 
@@ -86,6 +87,22 @@ class KnnServer:
         else:
             b += gsknn.work_bytes_s(N, nq, d, k)
         return b
+
+    def _fixed_bytes(self, k):
+        """The part of a batch's workspace that does not grow with its
+        queries (the candidate panel of the register-form filters: the whole
+        reference set in position order), taken as the workspace of one
+        query."""
+        if k not in self._fixed:
+            d, N = self.d, self.N
+            w = gsknn.bf16_work_bytes(N, 1, d, k) if self.mode == "bf16" else gsknn.work_bytes_s(N, 1, d, k)
+            self._fixed[k] = int(w)
+        return self._fixed[k]
+
+    def _share_bytes(self, nq, k):
+        """A request's share of a batch: its batch bytes less the fixed part,
+        which the budget holds once per stream instead."""
+        return max(0, self._batch_bytes(nq, k) - self._fixed_bytes(k))
 
     # -- one batch -----------------------------------------------------------
     def _launch(self, reqs, stream):
@@ -154,11 +171,14 @@ class KnnServer:
             now = time.perf_counter() - t0
             while future and (not realtime or future[0].arrival <= now):
                 waiting.append(future.popleft())
-            # admission by HBM budget (a request's share of a batch)
+            # admission by HBM budget: each request's share of a batch, plus
+            # the largest fixed part once per stream (a batch in flight per
+            # stream at most)
             while waiting:
                 r = waiting[0]
-                need = self._batch_bytes(r.queries.shape[0], r.k)
-                if self.reserved + need > self.budget and (self.reserved > 0 or inflight):
+                need = self._share_bytes(r.queries.shape[0], r.k)
+                fixed = len(self.streams) * max(self._fixed_bytes(kk) for kk in {r.k, *self._fixed})
+                if self.reserved + need + fixed > self.budget and (self.reserved > 0 or inflight):
                     break
                 waiting.popleft()
                 r.admitted = now

@@ -372,7 +372,9 @@ def test_bf16_carry_over_and_ties():
 # k = 700 takes the 2-wave filter shape and k = 1300 the 1-wave shape at
 # d = 24 (heaps too large for 4 waves of queries in LDS).
 @pytest.mark.parametrize("d,k,lattice", [(24, 17, False), (100, 32, False), (256, 64, False),
-                                         (12, 10, True), (24, 700, False), (24, 1300, False)])
+                                         (12, 10, True), (24, 700, False), (24, 1300, False),
+                                         (128, 1, False), (64, 65, False), (128, 100, False),
+                                         (33, 128, False), (16, 120, True), (128, 200, False)])
 def test_sgsknn_fp32_mfma_rerank_bitwise(d, k, lattice):
     # n >= 4096 takes the FP32 MFMA candidate pass + re-rank in the oracle's
     # FP32 arithmetic; distances must still be the oracle's bits.
@@ -765,3 +767,33 @@ def test_small_m_selection_ties(dtype, k, spread):
     for am in amaps:
         oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
     assert_rows(D.cpu().numpy(), I.cpu().numpy(), Dr, Ir)
+
+
+@pytest.mark.parametrize("warm", [0, 3, 1000])
+@pytest.mark.parametrize("k", [32, 100])
+def test_sgsknn_register_form_warm_up_and_carried_rows(warm, k, monkeypatch):
+    """The FP32 register-form filter (d <= 128, k <= 128) with warm-up passes
+    of 0 tiles, a few, and more than a split holds, on rows carried over from
+    a first call (resident distances seed the upper-bound sets): bitwise the
+    oracle's FP32 rows."""
+    import torch
+    monkeypatch.setenv("GSKNN_F32_WARM", str(warm))
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(4000 + warm + k))
+    N, d = 14000, 96
+    X = ds.low_rank(N, d, 12, seed=21).astype(np.float32)
+    X2 = oracle.sqnorm(X)
+    amaps = [g.integers(0, N, 900).astype(np.int32), g.integers(0, N, 7000).astype(np.int32)]
+    bmap = g.permutation(N)[:5000].astype(np.int32)
+    print(f"[gpu] sgsknn register form warm={warm} k={k}", flush=True)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    D, I = ds.empty_heap(bmap.size, k, np.float32)
+    tD, tI = torch.from_numpy(D).to(dev), torch.from_numpy(I).to(dev)
+    for am in amaps:
+        gsknn.sknn_dev(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2,
+                       torch.from_numpy(bmap).to(dev), tD, tI)
+    torch.cuda.synchronize()
+    Dr, Ir = D.copy(), I.copy()
+    for am in amaps:
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(tD.cpu().numpy(), tI.cpu().numpy(), Dr, Ir)
