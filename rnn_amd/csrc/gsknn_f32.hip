@@ -950,6 +950,10 @@ F32Plan plan_f32(int d, int k) {
     const char *e = getenv("GSKNN_F32_REG");
     return !(e && atoi(e) == 0);
   }();
+  // (At d = 256 the 64 KiB tiles leave one workgroup, one wave per SIMD,
+  // per CU: measured 110.5 / 137.5 ms at config-3 size for k = 64 / 100
+  // against 115.1 / 133.9 ms for the LDS-tile form, so the form stops at
+  // d = 128.)
   if (reg_on && d <= 128 && k <= 4 * kF32RegRK1) {
     p.reg = true;
     p.nb = d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 4 : 8;
