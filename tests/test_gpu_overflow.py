@@ -1,8 +1,9 @@
 """The re-scan fallback of the two-pass forms: when a query's candidate list
-(in one candidate split) outgrows its capacity (clamp(16 k, 512, 4096)
-entries for FP32; for the BF16 form's per-lane upper bounds, k <= 64 or
-k >= 48, clamp(32 k, 1024, 8192)), the re-rank re-scans every candidate of
-the split exactly.
+(in one candidate split) outgrows its capacity (the register forms with
+per-lane upper-bound sets -- FP32 at d <= 128 and k <= 128, BF16 at the k
+it covers -- list into four lane-group segments of clamp(32 k, 1024, 8192)
+/ 4 entries; the FP32 LDS-heap form holds clamp(16 k, 512, 4096)), the
+re-rank re-scans every candidate of the split exactly.
 Thousands of candidates at exactly the same distance from every query
 (duplicated rows with distinct indices) all reach the k-th boundary; with
 16384 queries a launch splits the candidates at most a few ways, so every
