@@ -581,6 +581,57 @@ def l1_norm(torch, dev, args):
     print(json.dumps(out), flush=True)
 
 
+def sgsknn_dims(torch, dev, args):
+    """sgsknn FP32 (the MFMA filter and the oracle-arithmetic re-rank) at the
+    dimensions of --dims and the k of --ks: low-rank points (latent 16,
+    N = 131072), amap / bmap the two halves of a seeded permutation
+    (m = n = 65536), empty rows.  HIP events around the whole call, median of
+    --steps calls after one warm-up."""
+    import numpy as np
+
+    from rnn_amd import datasets as ds
+    from rnn_amd import gsknn
+
+    stream = torch.cuda.current_stream()
+    N, m = 131072, 65536
+    out = {"workload": "sdims", "m": m, "n": m, "rows": []}
+    for d in [int(x) for x in args.dims.split(",")]:
+        X = ds.low_rank(N, d, 16, seed=4).astype(np.float32)
+        X2 = (X.astype(np.float64) ** 2).sum(1).astype(np.float32)
+        perm = np.random.Generator(np.random.PCG64(5)).permutation(N).astype(np.int32)
+        am, bm = torch.from_numpy(perm[:m]).to(dev), torch.from_numpy(perm[m:]).to(dev)
+        tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+        for k in [int(x) for x in args.ks.split(",")]:
+            work = torch.empty(max(16, gsknn.work_bytes_s(m, m, d, k)), dtype=torch.uint8, device=dev)
+            D = torch.empty((m, k), dtype=torch.float32, device=dev)
+            I = torch.empty((m, k), dtype=torch.int32, device=dev)
+
+            def call():
+                D.fill_(float("inf"))
+                I.fill_(-1)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                gsknn.sknn_dev(tX, tX2, am, tX, tX2, bm, D, I, work=work, stream=stream)
+                e1.record(stream)
+                return e0, e1
+
+            call()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(max(1, args.steps)):
+                e0, e1 = call()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = float(np.median(ts))
+            row = {"d": d, "k": k, "ms": ms, "pairs_per_s": m * float(m) / (ms * 1e-3),
+                   "tflops": 2.0 * d * m * m / (ms * 1e-3) / 1e12}
+            out["rows"].append(row)
+            log(f"sdims d={d} k={k}: {ms:.2f} ms, {row['tflops']:.1f} TFLOP/s")
+            del work, D, I
+    return out
+
+
 def dims_sweep(torch, dev, args):
     """dgsknn FP64 (AUTO) at other dimensions: the config-2 mixture generator
     at each d (N = 131072, 32 clusters, sigma 0.1), m = n = 65536 ids with
@@ -645,9 +696,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "l1", "dims", "copy"])
+                    choices=["c1", "c2", "c3", "c4", "c5", "l1", "dims", "sdims", "copy"])
     ap.add_argument("--dims", default="32,64,96,128,160,192,256,300,600",
-                    help="dimensions of the dims workload")
+                    help="dimensions of the dims and sdims workloads")
+    ap.add_argument("--ks", default="10,32,100", help="k of the sdims workload")
     ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
     ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
@@ -689,6 +741,9 @@ def main():
         return
     if args.workload == "dims":
         print(json.dumps(dims_sweep(torch, dev, args)), flush=True)
+        return
+    if args.workload == "sdims":
+        print(json.dumps(sgsknn_dims(torch, dev, args)), flush=True)
         return
     if args.workload == "c1":
         out = {"workload": "c1", **c1_small(torch, dev)}

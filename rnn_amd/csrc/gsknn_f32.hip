@@ -18,10 +18,12 @@
 // two FP32 epilogue additions on each side.  As in the BF16 form, U = k-th
 // smallest upper bound per query, and every candidate whose lower bound is <= U
 // when seen is listed.
-// Pass 2 (knn_f32_rerank_kernel) recomputes the listed candidates with the
-// oracle's arithmetic (one lane per candidate, p in order, no fused
-// multiply-add) and merges them into the FP32 heap row under the canonical
-// (distance, index) order: the distances are bit-identical to sgsknn_ref's.
+// Pass 2 recomputes the listed candidates with the oracle's arithmetic (one
+// lane per candidate, p in order, no fused multiply-add) and keeps the k
+// smallest of the row and the candidates under the canonical (distance,
+// index) order: the distances are bit-identical to sgsknn_ref's.  For k <= 256
+// a pool and rank selection do the merge (knn_f32_rerank_select_kernel),
+// above it an LDS heap (knn_f32_rerank_kernel).
 #include <limits.h>
 
 #include "gsknn_common.h"
@@ -879,9 +881,201 @@ __global__ __launch_bounds__(256) void knn_f32_rerank_kernel(F32Args a) {
   }
 }
 
+// The oracle's FP32 distance of candidate row xa and query row xb (p in
+// order from 0, every product and sum rounded), with the loads of 32 dims of
+// both rows issued together when the rows are 16-byte aligned (VEC).
+__device__ __forceinline__ float oracle_dist_f32v(const float *xa, const float *xb, int d, float xa2,
+                                                  float xb2, bool vec) {
+  float s = 0.0f;
+  int p = 0;
+  if (vec) {
+#pragma unroll 2
+    for (; p + 16 <= d; p += 16) {
+      f32x4 ca[4], cb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ca[u] = *(const f32x4 *)(xa + p + 4 * u);
+        cb[u] = *(const f32x4 *)(xb + p + 4 * u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s = __fadd_rn(s, __fmul_rn(ca[u].x, cb[u].x));
+        s = __fadd_rn(s, __fmul_rn(ca[u].y, cb[u].y));
+        s = __fadd_rn(s, __fmul_rn(ca[u].z, cb[u].z));
+        s = __fadd_rn(s, __fmul_rn(ca[u].w, cb[u].w));
+      }
+    }
+  }
+  for (; p < d; ++p) s = __fadd_rn(s, __fmul_rn(xa[p], xb[p]));
+  s = __fmul_rn(s, -2.0f);
+  s = __fadd_rn(s, xa2);
+  return __fadd_rn(s, xb2);
+}
+
+// Pass 2, selection form (k <= 64 (NS - 2)): one wave per query.  A pool in
+// LDS holds the resident row and every recomputed candidate that beats the
+// pool's k-th entry (the resident row's largest at first); when a batch of 64
+// would not fit, a rank selection keeps the k smallest under the canonical
+// (distance, index) order and gives the new k-th entry.  The last selection
+// writes the row in descending order (a valid max-heap, as the heap form
+// leaves it).  No lane runs heap operations alone.
+template <int NS>
+__global__ __launch_bounds__(256) void knn_f32_rerank_select_kernel(F32Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int P = 64 * NS;
+  const int k = a.k, d = a.d;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  float *pd = (float *)smem + (size_t)w * P;
+  int *pi = (int *)(smem + sizeof(float) * 4 * P) + (size_t)w * P;
+  float *rootd = (float *)(smem + sizeof(float) * 8 * P) + w;
+  int *rooti = (int *)(smem + sizeof(float) * 8 * P + 16) + w;
+  if (j >= a.n) return;  // the whole wave leaves; no block barrier below
+  auto wsync = []() __attribute__((always_inline)) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
+  const size_t row = (size_t)(a.hrow ? a.hrow[j] : j) * a.ldk;
+  float rd = -__builtin_inff();
+  int ri = INT_MIN;
+  for (int t = lane; t < k; t += 64) {
+    const float x = a.D[row + t];
+    const int xi = a.I[row + t];
+    pd[t] = x;
+    pi[t] = xi;
+    if (cl_f(rd, ri, x, xi)) {
+      rd = x;
+      ri = xi;
+    }
+  }
+  // The resident row's largest entry (no candidate at or above it can enter).
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float od = __shfl_xor(rd, off);
+    const int oi = __shfl_xor(ri, off);
+    if (cl_f(rd, ri, od, oi)) {
+      rd = od;
+      ri = oi;
+    }
+  }
+  int np = k;
+  // Keep the k smallest pool entries; the last call writes the row.
+  auto select = [&](bool last) __attribute__((always_inline)) {
+    wsync();
+    float od[NS];
+    int oi[NS], orank[NS];
+    const int ns = (np + 63) >> 6;
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const int e = lane + 64 * s2;
+      od[s2] = e < np ? pd[e] : 0.0f;
+      oi[s2] = e < np ? pi[e] : 0;
+      orank[s2] = 0;
+    }
+    for (int jj = 0; jj < np; ++jj) {
+      const float dj = pd[jj];
+      const int ij = pi[jj];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        if (s2 < ns) {
+          const int e = lane + 64 * s2;
+          const bool lt = dj < od[s2] || (dj == od[s2] && (ij < oi[s2] || (ij == oi[s2] && jj < e)));
+          orank[s2] += lt ? 1 : 0;
+        }
+      }
+    }
+    wsync();  // every rank taken before the pool is rewritten
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const int e = lane + 64 * s2;
+      if (e < np && orank[s2] < k) {
+        if (last) {
+          a.D[row + k - 1 - orank[s2]] = od[s2];
+          a.I[row + k - 1 - orank[s2]] = oi[s2];
+        } else {
+          pd[orank[s2]] = od[s2];
+          pi[orank[s2]] = oi[s2];
+          if (orank[s2] == k - 1) {
+            *rootd = od[s2];
+            *rooti = oi[s2];
+          }
+        }
+      }
+    }
+    wsync();
+    if (!last) {
+      rd = *rootd;
+      ri = *rooti;
+      np = k;
+    }
+  };
+
+  const bool vec = (d & 3) == 0 && ((uintptr_t)a.XA & 15) == 0 && ((uintptr_t)a.XB & 15) == 0;
+  const int qid = a.bmap[j];
+  const float *xb = a.XB + (size_t)qid * d;
+  const float xb2 = a.XB2[qid];
+  float Ulim = a.U[j];
+  for (int sp = 1; sp < a.nsplit; ++sp) Ulim = fminf(Ulim, a.U[(size_t)sp * a.n + j]);
+  const int capg = a.cap / a.G;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int sp = 0; sp < a.nsplit; ++sp) {
+    const size_t sj = (size_t)sp * a.n + j;
+    bool over = false;
+    for (int gg = 0; gg < a.G; ++gg) over |= a.cnt[((size_t)sp * a.G + gg) * a.n + j] > capg;
+    const int cbeg = sp * a.cps;
+    // An overflowed split is re-scanned once; otherwise each of its lists.
+    for (int gg = 0; gg < (over ? 1 : a.G); ++gg) {
+      const int nscan = over ? min(a.m, cbeg + a.cps) - cbeg : a.cnt[((size_t)sp * a.G + gg) * a.n + j];
+      const size_t lbase = sj * a.cap + (size_t)gg * capg;
+      for (int e0 = 0; e0 < nscan; e0 += 64) {
+        const int e = e0 + lane;
+        bool have = false;
+        int pos = 0;
+        if (e < nscan) {
+          if (over) {
+            pos = cbeg + e;
+            have = true;
+          } else {
+            const int2 en = a.lst[lbase + e];
+            pos = en.x;
+            have = __int_as_float(en.y) <= Ulim;
+          }
+        }
+        float dist = 0.0f;
+        int id = 0;
+        if (have) {
+          id = a.amap[pos];
+          dist = oracle_dist_f32v(a.XA + (size_t)id * d, xb, d, a.XA2[id], xb2, vec);
+        }
+        bool keep = have && cl_f(dist, id, rd, ri);
+        unsigned long long km = __ballot(keep);
+        if (np + __popcll(km) > P) {
+          select(false);
+          keep = keep && cl_f(dist, id, rd, ri);
+          km = __ballot(keep);
+        }
+        if (keep) {
+          const int slot = np + __popcll(km & below);
+          pd[slot] = dist;
+          pi[slot] = id;
+        }
+        np += __popcll(km);
+      }
+    }
+  }
+  select(true);
+}
+
 namespace {
 
 constexpr size_t kLdsF = 160 * 1024;
+
+// GSKNN_F32_SELECT=0 keeps the heap-form re-rank for every k (read at each
+// call; tuning only).
+bool f32_select_on() {
+  const char *e = getenv("GSKNN_F32_SELECT");
+  return !(e && atoi(e) == 0);
+}
 // Register form: upper-bound set size per lane, two query blocks per wave
 // for k <= 4 kF32RegRK, one (larger sets) for 4 kF32RegRK < k <= 4 kF32RegRK1.
 constexpr int kF32RegRK = 16;
@@ -1092,10 +1286,22 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
     default: return hipErrorNotSupported;
   }
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)knn_f32_rerank_kernel,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rl);
+  const int sel_ns = (a.k + 63) / 64 + 2;
+  const void *rk = (const void *)knn_f32_rerank_kernel;
+  size_t rlk = rl;
+  if (sel_ns <= 6 && f32_select_on()) {
+    const void *sk[4] = {(const void *)knn_f32_rerank_select_kernel<3>,
+                         (const void *)knn_f32_rerank_select_kernel<4>,
+                         (const void *)knn_f32_rerank_select_kernel<5>,
+                         (const void *)knn_f32_rerank_select_kernel<6>};
+    rk = sk[sel_ns - 3];
+    rlk = sizeof(float) * 8 * 64 * (size_t)sel_ns + 32;
+  }
+  e = hipFuncSetAttribute(rk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlk);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(knn_f32_rerank_kernel, dim3((a.n + 3) / 4), dim3(256), rl, s, a);
+  void *kargs[] = {(void *)&a};
+  e = hipLaunchKernel(rk, dim3((a.n + 3) / 4), dim3(256), kargs, rlk, s);
+  if (e != hipSuccess) return e;
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (overflow_out) e = hipMemcpyAsync(overflow_out, a.nover, sizeof(int), hipMemcpyDeviceToDevice, s);

@@ -797,3 +797,39 @@ def test_sgsknn_register_form_warm_up_and_carried_rows(warm, k, monkeypatch):
     for am in amaps:
         oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
     assert_rows(tD.cpu().numpy(), tI.cpu().numpy(), Dr, Ir)
+
+
+@pytest.mark.parametrize("d,k", [(96, 1), (96, 64), (128, 100), (40, 200), (40, 256)])
+def test_sgsknn_select_and_heap_rerank_agree(d, k, monkeypatch):
+    """The FP32 re-rank's selection form (k <= 256: a pool of recomputed
+    candidates, rank selection) and its heap form (GSKNN_F32_SELECT=0) on
+    rows carried over from a first call whose candidates repeat in the
+    second (identical (distance, index) pairs in the row and the lists):
+    the same bytes in the same order, and bitwise the oracle's FP32 rows."""
+    import torch
+    dev = torch.device("cuda")
+    g = np.random.Generator(np.random.PCG64(5100 + d + k))
+    N = 12000
+    X = ds.low_rank(N, d, 10, seed=23).astype(np.float32)
+    X2 = oracle.sqnorm(X)
+    a0 = g.integers(0, N, 3000).astype(np.int32)
+    amaps = [a0, np.concatenate([a0[:1500], g.integers(0, N, 6000).astype(np.int32)])]
+    bmap = g.permutation(N)[:4000].astype(np.int32)
+    tX, tX2 = torch.from_numpy(X).to(dev), torch.from_numpy(X2).to(dev)
+    rows = {}
+    for sel in ("1", "0"):
+        monkeypatch.setenv("GSKNN_F32_SELECT", sel)
+        print(f"[gpu] sgsknn re-rank select={sel} d={d} k={k}", flush=True)
+        D, I = ds.empty_heap(bmap.size, k, np.float32)
+        tD, tI = torch.from_numpy(D).to(dev), torch.from_numpy(I).to(dev)
+        for am in amaps:
+            gsknn.sknn_dev(tX, tX2, torch.from_numpy(am).to(dev), tX, tX2,
+                           torch.from_numpy(bmap).to(dev), tD, tI)
+        torch.cuda.synchronize()
+        rows[sel] = (tD.cpu().numpy(), tI.cpu().numpy())
+    assert np.array_equal(rows["1"][0].view(np.uint32), rows["0"][0].view(np.uint32))
+    np.testing.assert_array_equal(rows["1"][1], rows["0"][1])
+    Dr, Ir = ds.empty_heap(bmap.size, k, np.float32)
+    for am in amaps:
+        oracle.gsknn(X, X2, am, X, X2, bmap, Dr, Ir, canonical=True)
+    assert_rows(rows["1"][0], rows["1"][1], Dr, Ir)
