@@ -476,18 +476,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void kn
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (lane < Q && q0 + lane < L_) {
-      // Heapsort ascending, then write descending into the row.
-      for (int e = k - 1; e > 0; --e) {
-        const float xd = hd[e * Q + lane];
-        const int xi = hi[e * Q + lane];
-        hd[e * Q + lane] = hd[lane];
-        hi[e * Q + lane] = hi[lane];
-        sift_down(hd, hi, Q, lane, 0, e, xd, xi);
-      }
+      // The row is stored as the max-heap it is (the reference's row
+      // format; the next iteration's load and the sorted-output call take
+      // any heap order).
       const size_t row = (size_t)leaf[q0 + lane] * a.ldk;
       for (int t = 0; t < k; ++t) {
-        a.D[row + t] = hd[(k - 1 - t) * Q + lane];
-        a.I[row + t] = hi[(k - 1 - t) * Q + lane];
+        a.D[row + t] = hd[t * Q + lane];
+        a.I[row + t] = hi[t * Q + lane];
       }
     }
     __builtin_amdgcn_wave_barrier();
