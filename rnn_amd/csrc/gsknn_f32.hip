@@ -114,20 +114,46 @@ struct F32Args {
 // Norms of the gathered points (FP64 sum, rounded up) and candidate maxima.
 __global__ void f32_prep_kernel(F32Args a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.m) {
-    const float *x = a.XA + (size_t)a.amap[i] * a.d;
+  const bool vec = (a.d & 3) == 0 && (((uintptr_t)a.XA | (uintptr_t)a.XB) & 15) == 0;
+  // |x| in FP64, p in order, rounded up; rows read 16 bytes at a time when
+  // d is a multiple of 4 and the tables are 16-byte aligned.
+  auto norm = [&](const float *x) -> float {
     double s = 0.0;
-    for (int p = 0; p < a.d; ++p) s += (double)x[p] * (double)x[p];
-    const float xn = __double2float_ru(sqrt(s) * (1.0 + 0x1p-40));
+    int p = 0;
+    if (vec) {
+      for (; p < a.d; p += 4) {
+        const f32x4 v = *(const f32x4 *)(x + p);
+        s += (double)v.x * (double)v.x;
+        s += (double)v.y * (double)v.y;
+        s += (double)v.z * (double)v.z;
+        s += (double)v.w * (double)v.w;
+      }
+    }
+    for (; p < a.d; ++p) s += (double)x[p] * (double)x[p];
+    return __double2float_ru(sqrt(s) * (1.0 + 0x1p-40));
+  };
+  // Bit patterns of this lane's candidates for the table maxima (the values
+  // are >= 0 or NaN, so their bits order as integers).
+  int m0 = 0, m1 = 0;
+  if (i < a.m) {
+    const int id = a.amap[i];
+    const float xn = norm(a.XA + (size_t)id * a.d);
     a.xnA[i] = xn;
-    atomicMax((int *)&a.mx[0], __float_as_int(xn));
-    atomicMax((int *)&a.mx[1], __float_as_int(fabsf(a.XA2[a.amap[i]])));
+    m0 = __float_as_int(xn);
+    m1 = __float_as_int(fabsf(a.XA2[id]));
   } else if (i < a.m + a.n) {
     const int j = i - a.m;
-    const float *x = a.XB + (size_t)a.bmap[j] * a.d;
-    double s = 0.0;
-    for (int p = 0; p < a.d; ++p) s += (double)x[p] * (double)x[p];
-    a.xnB[j] = __double2float_ru(sqrt(s) * (1.0 + 0x1p-40));
+    a.xnB[j] = norm(a.XB + (size_t)a.bmap[j] * a.d);
+  }
+  // Wave maxima first, then one atomic per wave instead of one per row.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = max(m0, __shfl_xor(m0, off));
+    m1 = max(m1, __shfl_xor(m1, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (m0 > 0) atomicMax((int *)&a.mx[0], m0);
+    if (m1 > 0) atomicMax((int *)&a.mx[1], m1);
   }
 }
 
