@@ -499,7 +499,7 @@ __attribute__((amdgpu_waves_per_eu(f32_waves_per_eu(KS, QB)))) void knn_f32_filt
 }
 
 // ---------------------------------------------------------------------------
-// Register form of pass 1 (d <= 128, k <= 64), the structure of the BF16
+// Register form of pass 1 (d <= 256, k <= 128), the structure of the BF16
 // register-form filter on FP32 operands: candidates gathered into a panel in
 // position order (FP32 rows zero padded to DP = 16 NB dims and a record
 // {XA2, |a| rounded up} per position), tiles of 64 candidates loaded straight
@@ -1163,24 +1163,24 @@ F32Plan plan_f32(int d, int k) {
   }
   const int cap = 16 * k;
   p.cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
-  // Register form for d <= 128 and k <= 4 kF32RegRK1 (GSKNN_F32_REG=0 keeps
+  // Register form for d <= 256 and k <= 4 kF32RegRK1 (GSKNN_F32_REG=0 keeps
   // the LDS-tile form, for comparison); its lists get twice the capacity,
   // in four lane-group segments.
   static const bool reg_on = [] {
     const char *e = getenv("GSKNN_F32_REG");
     return !(e && atoi(e) == 0);
   }();
-  // (At d = 256 the 64 KiB tiles leave one workgroup, one wave per SIMD,
-  // per CU: measured 110.5 / 137.5 ms at config-3 size for k = 64 / 100
-  // against 115.1 / 133.9 ms for the LDS-tile form, so the form stops at
-  // d = 128.)
-  if (reg_on && d <= 128 && k <= 4 * kF32RegRK1) {
+  // Above d = 128 the 64 KiB tiles (two buffers) leave one workgroup per
+  // CU, so it takes eight waves (two per SIMD) sharing each tile: at
+  // config-3 size (d = 256, m = n = 131072, k = 64) 81 ms against 110 for the
+  // LDS-tile form (four waves: 110.5).
+  if (reg_on && d <= 256 && k <= 4 * kF32RegRK1) {
     p.reg = true;
-    p.nb = d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 4 : 8;
+    p.nb = d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 4 : d <= 128 ? 8 : 16;
     p.qb = k <= 4 * kF32RegRK ? 2 : 1;
     p.cb = 4;
-    p.w = 4;
-    p.q = 4 * 16 * p.qb;
+    p.w = p.nb == 16 ? 8 : 4;
+    p.q = p.w * 16 * p.qb;
     p.lds = F32RegLayout(p.nb, 4).bytes;
     p.G = 4;
     const int c2 = 32 * k;
@@ -1297,11 +1297,13 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
       case 2 * 4 + 1: kern = knn_f32_filter2_kernel<2, 1, 4, 4, kF32RegRK1>; break;
       case 4 * 4 + 1: kern = knn_f32_filter2_kernel<4, 1, 4, 4, kF32RegRK1>; break;
       case 8 * 4 + 1: kern = knn_f32_filter2_kernel<8, 1, 4, 4, kF32RegRK1>; break;
+      case 16 * 4 + 2: kern = knn_f32_filter2_kernel<16, 2, 4, 8, kF32RegRK>; break;
+      case 16 * 4 + 1: kern = knn_f32_filter2_kernel<16, 1, 4, 8, kF32RegRK1>; break;
       default: return hipErrorNotSupported;
     }
     e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(256), p.lds, s, a);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * p.w), p.lds, s, a);
     e = hipGetLastError();
   } else switch (p.ks) {
     case 4: e = launch_f32_ks<4>(p, a, s); break;

@@ -374,7 +374,8 @@ def test_bf16_carry_over_and_ties():
 @pytest.mark.parametrize("d,k,lattice", [(24, 17, False), (100, 32, False), (256, 64, False),
                                          (12, 10, True), (24, 700, False), (24, 1300, False),
                                          (128, 1, False), (64, 65, False), (128, 100, False),
-                                         (33, 128, False), (16, 120, True), (128, 200, False)])
+                                         (33, 128, False), (16, 120, True), (128, 200, False),
+                                         (200, 100, False), (129, 10, False)])
 def test_sgsknn_fp32_mfma_rerank_bitwise(d, k, lattice):
     # n >= 4096 takes the FP32 MFMA candidate pass + re-rank in the oracle's
     # FP32 arithmetic; distances must still be the oracle's bits.
@@ -772,7 +773,7 @@ def test_small_m_selection_ties(dtype, k, spread):
 @pytest.mark.parametrize("warm", [0, 3, 1000])
 @pytest.mark.parametrize("k", [32, 100])
 def test_sgsknn_register_form_warm_up_and_carried_rows(warm, k, monkeypatch):
-    """The FP32 register-form filter (d <= 128, k <= 128) with warm-up passes
+    """The FP32 register-form filter (d <= 256, k <= 128) with warm-up passes
     of 0 tiles, a few, and more than a split holds, on rows carried over from
     a first call (resident distances seed the upper-bound sets): bitwise the
     oracle's FP32 rows."""

@@ -1,6 +1,6 @@
 """The re-scan fallback of the two-pass forms: when a query's candidate list
 (in one candidate split) outgrows its capacity (the register forms with
-per-lane upper-bound sets -- FP32 at d <= 128 and k <= 128, BF16 at the k
+per-lane upper-bound sets -- FP32 at d <= 256 and k <= 128, BF16 at the k
 it covers -- list into four lane-group segments of clamp(32 k, 1024, 8192)
 / 4 entries; the FP32 LDS-heap form holds clamp(16 k, 512, 4096)), the
 re-rank re-scans every candidate of the split exactly.
