@@ -365,7 +365,7 @@ def serving(torch, dev, args):
            "rate_req_per_s": args.rate, "offered_pairs_per_s": nq * N / base[-1].arrival}
     results = {}
     for mode in ("bf16", "fp32"):
-        srv = serve.KnnServer(R, mode=mode, hbm_budget=32 << 30, max_batch=16384, streams=3)
+        srv = serve.KnnServer(R, mode=mode, hbm_budget=32 << 30, max_batch=args.max_batch, streams=3)
         warm = [serve.Request(-1, 0.0, base[0].queries[:8], kk) for kk in (10, 32, 100)]
         srv.run(warm, realtime=False)
         srv.batch_sizes.clear()
@@ -702,6 +702,7 @@ def main():
     ap.add_argument("--ks", default="10,32,100", help="k of the sdims workload")
     ap.add_argument("--requests", type=int, default=4096, help="serving trace length (c5)")
     ap.add_argument("--rate", type=float, default=200.0, help="arrivals per second (c5)")
+    ap.add_argument("--max-batch", type=int, default=16384, help="queries per serving batch at most (c5)")
     ap.add_argument("--iters", type=int, default=8, help="tree iterations (c4)")
     ap.add_argument("--scale", type=int, default=1, help="divide the workload sizes (c3)")
     ap.add_argument("--no-cpu", action="store_true")
