@@ -1275,10 +1275,14 @@ hipError_t launch_knn_f32_mfma(const KnnArgs<float> &ka, hipStream_t s, void *wo
   a.nover = (int *)(wb + W.nover);
   a.G = p.G;
   a.panel = (const unsigned char *)(wb + W.panel);
-  // Warm-up pass of the register form: the first 1/16 of each split's tiles
-  // (GSKNN_F32_WARM=<tiles>, read at each call, sets the count).
+  // Warm-up pass of the register form: the first 1/16 of each split's tiles,
+  // at most 32 (GSKNN_F32_WARM=<tiles>, read at each call, sets the count).
+  // Longer passes cost more MFMA work than the list entries they save: at
+  // m = n = 65536, d = 128, k = 32, 0 / 32 / 64 / 128 tiles took 10.71 /
+  // 10.71 / 10.92 / 11.42 ms, and serving over 2M candidates (about 290 tiles
+  // per split at 1/16) ran p50 216 ms with 32 against 271.
   const char *warm_e = getenv("GSKNN_F32_WARM");
-  a.warm = p.reg ? (warm_e ? atoi(warm_e) : p.cps / 64 / 16) : 0;
+  a.warm = p.reg ? (warm_e ? atoi(warm_e) : min(p.cps / 64 / 16, 32)) : 0;
   hipLaunchKernelGGL(f32_zero_kernel, dim3(1), dim3(64), 0, s, a.mx, a.nover);
   hipLaunchKernelGGL(f32_prep_kernel, dim3((a.m + a.n + 255) / 256), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
