@@ -1636,8 +1636,17 @@ Bf16Plan plan_bf16(int d, int k) {
       Filter2Layout(p.ks, kCBb).bytes <= kLds) {
     p.reg = true;
     p.qb = k <= 4 * kRegRK ? 2 : 1;
-    p.w = 4;
-    p.q = 4 * 16 * p.qb;
+    // At d > 192 (KS = 8) eight waves share each tile: the two 32 KiB tile
+    // buffers leave two workgroups per CU, so four waves per SIMD instead of
+    // two hide the survivor loop's latency (config 3: 14.6 / 14.8 ms against
+    // 16.1 / 16.2 on one box; GSKNN_BF16_W8=0 keeps four waves, for
+    // comparison).
+    static const bool w8 = [] {
+      const char *e = getenv("GSKNN_BF16_W8");
+      return !(e && atoi(e) == 0);
+    }();
+    p.w = w8 && p.ks == 8 ? 8 : 4;
+    p.q = p.w * 16 * p.qb;
     p.lds = Filter2Layout(p.ks, kCBb).bytes;
     p.G = 4;
   }
@@ -1686,10 +1695,15 @@ hipError_t launch_filter_ks(const Bf16Plan &p, const Bf16Args &a, hipStream_t s)
   if (p.reg) {
     auto kern = p.qb == 2 ? knn_bf16_filter2_kernel<KS, 2, kCBb, 4, kRegRK>
                           : knn_bf16_filter2_kernel<KS, 1, kCBb, 4, kRegRK1>;
+    if constexpr (KS == 8) {
+      if (p.w == 8)
+        kern = p.qb == 2 ? knn_bf16_filter2_kernel<KS, 2, kCBb, 8, kRegRK>
+                         : knn_bf16_filter2_kernel<KS, 1, kCBb, 8, kRegRK1>;
+    }
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)p.lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(256), p.lds, s, a);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * p.w), p.lds, s, a);
     return hipGetLastError();
   }
 #define GSKNN_BF16_CASE(QB_, W_)                                                       \
